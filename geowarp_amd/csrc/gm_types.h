@@ -1,0 +1,85 @@
+// Plain-old-data argument blocks shared by the C ABI (include/geowarp_amd.h)
+// and the HIP kernels.  Pointers are device pointers; every array is
+// row-major.  Nodal vectors are node-interleaved: value k of node n sits at
+// n*nb + k (nb = dofs per node).
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gm_grid {
+  int dim;       /* 2 or 3 */
+  int nx, ny, nz; /* background cells per axis (nz = 0 in 2D) */
+  int n_nodes;   /* (nx+1)(ny+1)(nz+1) */
+  int nb;        /* dofs per node: dim (single phase) or dim+1 (u-p) */
+  double h[3];   /* spacing used by the shape functions (2D: dx on both axes) */
+  double inv_h[3];
+} gm_grid;
+
+/* Per-step particle binning: particles sorted by the node index of their
+   stencil base (lo_x, lo_y[, lo_z]). */
+typedef struct gm_bins {
+  int n_particles;
+  int* cell_start;   /* n_nodes + 1 */
+  int* cell_count;   /* n_nodes + 1 (scratch) */
+  int* sorted;       /* n_particles: particle ids ordered by base node, then id */
+  uint8_t* ext;      /* n_particles: bit a set => 3 nodes along axis a, else 2 */
+  int* scan_tmp;     /* >= 1024 ints of scratch */
+} gm_bins;
+
+/* Per-step block-CSR pattern over nodes (5^dim window around each node). */
+typedef struct gm_pattern {
+  int* row_ptr;       /* n_nodes + 1 */
+  int* col_idx;       /* nnzb */
+  uint64_t* row_mask; /* n_nodes * 2: window-offset bitmask of each row */
+  int nnzb;
+} gm_pattern;
+
+/* Arguments of the per-particle stress/tangent stage. */
+typedef struct gm_stage_args {
+  int n_particles;
+  int coupled;       /* 0: single phase, 1: u-p */
+  int want_tangent;  /* write the particle tangent D */
+  int tangent_fp32;  /* store D as float instead of double */
+  const double* x;   /* n_p x dim */
+  const double* lp;  /* n_p x dim */
+  const double* F_old; double* F_new;   /* n_p x 9 */
+  const double* b_old; double* b_new;   /* n_p x 9 */
+  double* sigma;                        /* n_p x 9 */
+  const double* alpha_old; double* alpha_new; /* n_p (J2 hardening), may be null */
+  const uint8_t* ext_flag;   /* n_p point-load flags, may be null */
+  const uint8_t* trac_flag;  /* n_p traction flags, may be null */
+  const double* u_new;  /* single phase: increment; u-p: new solution (node-interleaved) */
+  const double* u_old;  /* u-p: old solution (node-interleaved) */
+  double* Y;            /* n_p x nY particle "stress vector" */
+  void* D;              /* n_p x nY x nZ tangent (double or float) */
+  double* pen_force;    /* n_p per-particle indentation force (3D), may be null */
+  /* material */
+  int mat_kind; int dp_apex;
+  double lam, mu, E, kappa, H, phi, psi, coh, shape;
+  /* loading / porous media */
+  double p_vol, p_rho, phi0, mobility, dt, PPD;
+  double gravity[3];     /* already ramped by (step+1)/total */
+  double point_load[3];  /* already ramped */
+  double traction[3];    /* per-particle line load already multiplied by h/PPD */
+  double footing[4];     /* min_x, max_x, min_y, max_y */
+  double footing_height; /* current indenter height */
+  double pen_par;        /* penalty stiffness E*pf*(dx/PPD)(dy/PPD) */
+  int penalty;           /* 3D indenter active */
+} gm_stage_args;
+
+/* Krylov workspace (device pointers).  Vectors hold n_rows*nb entries of the
+   solve precision; dinv holds n_rows*nb*nb; part >= 2*max(n_rows/32+1, 1024)
+   doubles; sc >= 16 doubles. */
+typedef struct gm_krylov_ws {
+  void* r; void* rhat; void* p; void* v; void* s; void* t; void* y; void* z;  /* n_rows*nb each, TV */
+  void* dinv;     /* n_rows*nb*nb, TM */
+  double* part;   /* >= 2*max(spmv blocks, 1024) */
+  double* sc;     /* >= 16 */
+} gm_krylov_ws;
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,233 @@
+"""Shared implementation of the three quasi-static solver classes.
+
+The public classes (quasi_static_solver_2d, quasi_static_coupled_solver_2d,
+quasi_static_coupled_solver_3d) keep the reference constructor signatures,
+dict keys, method names and state attributes (solvers/quasi_static_*.py);
+their arrays are torch tensors on the ROCm device, FP64, in the reference
+shapes, with dof-major nodal vectors.  Every numerical phase runs in
+libgeowarp_amd.so; this module only sequences the calls.
+
+Extensions over the reference (all optional, defaults reproduce it):
+  iteration_dict['tol_mode']      'absolute' (reference, qs2d:230) or 'relative'
+  iteration_dict['precision']     'fp64' (default) or 'mixed' (FP32 Jacobian +
+                                  FP32 Krylov, FP64 residual/refinement)
+  iteration_dict['linear_tol'], ['linear_maxiter'], ['ir_steps']
+  iteration_dict['verbose']       print residual norms like the reference
+  material_dict['H']              J2 linear isotropic hardening modulus
+  material_dict['friction_angle'], ['dilation_angle'], ['cohesion'],
+  ['dp_apex']                     Drucker-Prager ('reference' or 'return')
+"""
+import math
+
+import numpy as np
+import torch
+
+from .. import _lib as L
+from ._engine import Engine, MAT_KIND
+
+
+class DofStruct:
+    """DofStruct (qs2d:40-43): activation and Dirichlet flags, dof-major bool."""
+
+    def __init__(self, n, device):
+        self.activate_flag_array = torch.zeros(n, dtype=torch.bool, device=device)
+        self.boundary_flag_array = torch.zeros(n, dtype=torch.bool, device=device)
+
+
+def init_particles_rectangle(dim, start, end, dx, n_grid, PPD):
+    """init_particles_rectangle_2d/3d (mpm_utils.py:12-51) without Python loops:
+    cells whose centre lies strictly inside (start, end), PPD^d points per cell,
+    ordered by (i, j[, k], a, b[, c])."""
+    axes = []
+    for a in range(dim):
+        c = (np.arange(n_grid[a]) + 0.5) * dx
+        axes.append(np.nonzero((start[a] < c) & (c < end[a]))[0])
+    sub = (0.5 + np.arange(PPD)) * dx / PPD
+    if dim == 2:
+        I, J, A, B = np.meshgrid(axes[0], axes[1], np.arange(PPD), np.arange(PPD), indexing='ij')
+        x = np.stack([I * dx + sub[A], J * dx + sub[B]], axis=-1)
+    else:
+        I, J, K, A, B, C = np.meshgrid(axes[0], axes[1], axes[2], np.arange(PPD), np.arange(PPD), np.arange(PPD),
+                                       indexing='ij')
+        x = np.stack([I * dx + sub[A], J * dx + sub[B], K * dx + sub[C]], axis=-1)
+    return x.reshape(-1, dim).astype(np.float64)
+
+
+class _QuasiStaticBase:
+    dim = 2
+    coupled = False
+
+    # ------------------------------------------------------------------ construction
+    def _common_init(self, background_grid_dict, particles_dict, iteration_dict, material_dict, load_dict,
+                     device, x_particles):
+        g = background_grid_dict
+        self.n_grid_x = g['n_grid_x']
+        self.n_grid_y = g['n_grid_y']
+        self.dx = g['dx']
+        self.inv_dx = 1.0 / self.dx
+        self.max_x = g.get('max_x')
+        self.start_x, self.end_x = g['start_x'], g['end_x']
+        self.start_y, self.end_y = g['start_y'], g['end_y']
+        self.domain_min_x, self.domain_max_x = g['domain_min_x'], g['domain_max_x']
+        self.domain_min_y, self.domain_max_y = g['domain_min_y'], g['domain_max_y']
+        if self.dim == 3:
+            self.n_grid_z = g['n_grid_z']
+            self.n_nodes = (self.n_grid_x + 1) * (self.n_grid_y + 1) * (self.n_grid_z + 1)
+        else:
+            self.n_nodes = (self.n_grid_x + 1) * (self.n_grid_y + 1)
+        nb = self.dim + (1 if self.coupled else 0)
+        self.n_matrix_size = nb * self.n_nodes
+        p = particles_dict
+        self.PPD = p['PPD']
+        self.p_vol = p['p_vol']
+        self.p_rho = p['p_rho']
+        it = iteration_dict
+        self.n_iter = it['n_iter']
+        self.tol = it['tol']
+        self.solver_name = it.get('solver_name', 'warp')
+        self.tol_mode = it.get('tol_mode', 'absolute')
+        self.precision = it.get('precision', 'fp64')
+        self.linear_tol = it.get('linear_tol', 1e-10 if self.precision == 'fp64' else 1e-4)
+        self.linear_maxiter = it.get('linear_maxiter', 20000)
+        self.ir_steps = it.get('ir_steps', 1)
+        self.verbose = it.get('verbose', False)
+        m = material_dict
+        self.material_name = m['material_name']
+        if self.material_name not in MAT_KIND:
+            raise ValueError("Unimplemented material model: %s" % self.material_name)
+        self.youngs_modulus = m['youngs_modulus']
+        self.poisson_ratio = m['poisson_ratio']
+        self._set_lame()
+        self.kappa = m.get('kappa', 0.0)
+        self.hardening = m.get('H', 0.0)
+        self.friction_angle = m.get('friction_angle', 30.0)
+        self.dilation_angle = m.get('dilation_angle', 0.0)
+        self.cohesion = m.get('cohesion', 0.0)
+        self.shape_factor = m.get('shape_factor', 0.0)
+        self.dp_apex = {'reference': 0, 'return': 1}[m.get('dp_apex', 'reference')]
+        self.gravity_mag = load_dict['gravity_mag']
+        self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
+        spacing = (self.dx, self.dx, 0.0) if self.dim == 2 else (self.dx, self.dy, self.dz)
+        n_grid = (self.n_grid_x, self.n_grid_y) if self.dim == 2 else (self.n_grid_x, self.n_grid_y, self.n_grid_z)
+        self.engine = Engine(self.dim, self.coupled, n_grid, spacing, device=self.device, precision=self.precision)
+        dev = self.device
+        self.dofStruct = DofStruct(self.n_matrix_size, dev)
+        if x_particles is None:
+            start = (self.start_x, self.start_y) + ((self.start_z,) if self.dim == 3 else ())
+            end = (self.end_x, self.end_y) + ((self.end_z,) if self.dim == 3 else ())
+            x_particles = init_particles_rectangle(self.dim, start, end, self.dx, n_grid, self.PPD)
+        x_np = np.asarray(x_particles, dtype=np.float64).reshape(-1, self.dim)
+        self.n_particles = x_np.shape[0]
+        if 'n_particles' in p and p['n_particles'] != self.n_particles:
+            raise ValueError('particles_dict n_particles=%d but the lattice holds %d' % (p['n_particles'],
+                                                                                         self.n_particles))
+        f64 = dict(dtype=torch.float64, device=dev)
+        n = self.n_particles
+        self.x_particles = torch.tensor(x_np, **f64)
+        self.x0 = self.x_particles.clone()
+        self.deformation_gradient_total_new = torch.zeros(n, 3, 3, **f64)
+        self.deformation_gradient_total_old = torch.zeros(n, 3, 3, **f64)
+        self.left_Cauchy_Green_new = torch.zeros(n, 3, 3, **f64)
+        self.left_Cauchy_Green_old = torch.zeros(n, 3, 3, **f64)
+        self.particle_Cauchy_stress_array = torch.zeros(n, 3, 3, **f64)
+        self.plastic_strain_new = torch.zeros(n, **f64)
+        self.plastic_strain_old = torch.zeros(n, **f64)
+        self.GIMP_lp = torch.zeros(n, self.dim, **f64)
+        self.particle_external_flag_array = torch.zeros(n, dtype=torch.bool, device=dev)
+        self.residual_history = []
+        self.newton_iterations = 0
+        self._initialization()
+
+    def _set_lame(self):
+        E, nu = self.youngs_modulus, self.poisson_ratio
+        self.lame_lambda = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+        self.lame_mu = E / (2.0 * (1.0 + nu))
+
+    def _initialization(self):
+        """initialization (mpm_utils.py:53-74) + initialize_GIMP_lp (gimp.py:8-22)."""
+        I = torch.diag(torch.tensor([1.0 + 1e-7, 1.0 - 1e-6, 1.0], dtype=torch.float64, device=self.device))
+        for t in (self.deformation_gradient_total_new, self.deformation_gradient_total_old,
+                  self.left_Cauchy_Green_new, self.left_Cauchy_Green_old):
+            t.copy_(I.expand_as(t))
+        self.plastic_strain_new.zero_()
+        self.plastic_strain_old.zero_()
+        self.GIMP_lp.fill_(self.GIMP_lp_initial)
+
+    # ------------------------------------------------------------------ stage arguments
+    def _stage_args(self, current_step, total_steps, u_new, u_old):
+        a = L.gm_stage_args()
+        a.n_particles = self.n_particles
+        a.coupled = int(self.coupled)
+        a.x = self.x_particles.data_ptr()
+        a.lp = self.GIMP_lp.data_ptr()
+        a.F_old = self.deformation_gradient_total_old.data_ptr()
+        a.F_new = self.deformation_gradient_total_new.data_ptr()
+        a.b_old = self.left_Cauchy_Green_old.data_ptr()
+        a.b_new = self.left_Cauchy_Green_new.data_ptr()
+        a.sigma = self.particle_Cauchy_stress_array.data_ptr()
+        a.alpha_old = self.plastic_strain_old.data_ptr()
+        a.alpha_new = self.plastic_strain_new.data_ptr()
+        self._ext_u8 = self.particle_external_flag_array.to(torch.uint8)
+        a.ext_flag = self._ext_u8.data_ptr()
+        a.trac_flag = None
+        a.u_new = u_new.data_ptr()
+        a.u_old = u_old.data_ptr() if u_old is not None else None
+        a.pen_force = self.engine.pen_force.data_ptr() if self.engine.n_particles > 0 else None
+        a.mat_kind = MAT_KIND[self.material_name]
+        a.dp_apex = self.dp_apex
+        a.lam, a.mu, a.E = self.lame_lambda, self.lame_mu, self.youngs_modulus
+        a.kappa, a.H = self.kappa, self.hardening
+        a.phi, a.psi, a.coh, a.shape = self.friction_angle, self.dilation_angle, self.cohesion, self.shape_factor
+        a.p_vol, a.p_rho = self.p_vol, self.p_rho
+        a.PPD = float(self.PPD)
+        s1 = (float(current_step) + 1.0) / float(total_steps)
+        g = [0.0, 0.0, 0.0]
+        g[self.dim - 1] = -self.gravity_mag * s1
+        for k in range(3):
+            a.gravity[k] = g[k]
+        return a
+
+    # ------------------------------------------------------------------ Newton loop
+    def _newton(self, current_step, total_steps, u_new, u_old, args, after_residual=None):
+        """newton_iter of the reference (qs2d:151-231): per iteration residual,
+        Jacobian, solve, u += delta, then the test on the pre-update norm."""
+        eng = self.engine
+        x, lp = self.x_particles, self.GIMP_lp
+        self.residual_history = []
+        r0 = None
+        it = 0
+        for it in range(self.n_iter):
+            eng.stage(args, want_tangent=True)
+            eng.residual(x, lp)
+            eng.norm2(eng.rhs, 0)
+            if after_residual is not None:
+                after_residual()
+            eng.jacobian(x, lp)
+            eng.solve(self.linear_tol, self.linear_maxiter, ir_steps=self.ir_steps)
+            eng.axpy(1.0, eng.delta, u_new)
+            nr = math.sqrt(max(eng.scal[0].item(), 0.0))
+            self.residual_history.append(nr)
+            if self.verbose:
+                print('residual.norm:', nr)
+            if not math.isfinite(nr):
+                raise L.GMError('Newton residual is not finite at iteration %d' % it)
+            if r0 is None:
+                r0 = nr
+            crit = nr if self.tol_mode == 'absolute' else (nr / r0 if r0 > 0 else 0.0)
+            if crit < self.tol:
+                break
+        self.newton_iterations = it + 1
+
+    # ------------------------------------------------------------------ dof-major views
+    def _sync_flags_out(self):
+        eng = self.engine
+        self.dofStruct.activate_flag_array.copy_(eng.dofmajor(eng.act).bool())
+        self.dofStruct.boundary_flag_array.copy_(eng.dofmajor(eng.bflag).bool())
+
+    @property
+    def rhs(self):
+        return self.engine.dofmajor(self.engine.rhs)
+
+    @property
+    def increment_iteration(self):
+        return self.engine.dofmajor(self.engine.delta)

@@ -1,0 +1,339 @@
+"""Device engine behind the three solver classes.
+
+Owns every device buffer of one background grid + particle set and runs the
+load-step phases through the C ABI (include/geowarp_amd.h):
+
+  begin_step      bin particles by stencil base, build the per-step block-CSR
+                  pattern (weights are fixed inside a load step)
+  p2g_*           activation / pressure-Dirichlet flags / consistent mass
+  newton          residual (FP64) -> Jacobian (FP64 or FP32) -> Krylov -> update
+  g2p, update_lp
+
+Nodal vectors are node-interleaved internally (value k of node n at n*nb+k);
+the solver classes expose the reference's dof-major views.
+"""
+import math
+import time
+
+import torch
+
+from .. import _lib as L
+
+MAT_KIND = {'Hencky elasticity': 0, 'J2': 1, 'Neo-Hookean': 2, 'Drucker-Prager': 3}
+
+
+def _stream():
+    return L.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _layout(dim, coupled):
+    nh = dim * dim
+    if not coupled:
+        return nh + dim, nh  # NY, NZ
+    nz = nh + (1 if dim == 3 else 0) + 2 + dim
+    ny = nh + dim + 1 + dim + 1
+    return ny, nz
+
+
+class Engine:
+    def __init__(self, dim, coupled, n_grid, h, device=None, precision='fp64'):
+        L.lib()
+        if device is None:
+            if not torch.cuda.is_available():
+                raise L.GMError('geowarp_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
+            device = torch.device('cuda', torch.cuda.current_device())
+        self.dev = torch.device(device)
+        if self.dev.type != 'cuda':
+            raise L.GMError('geowarp_amd runs on a ROCm device only')
+        if precision not in ('fp64', 'mixed'):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.dim = dim
+        self.coupled = bool(coupled)
+        self.nb = dim + (1 if coupled else 0)
+        nx, ny = n_grid[0], n_grid[1]
+        nz = n_grid[2] if dim == 3 else 0
+        self.n_nodes = (nx + 1) * (ny + 1) * ((nz + 1) if dim == 3 else 1)
+        g = L.gm_grid()
+        g.dim, g.nx, g.ny, g.nz, g.n_nodes, g.nb = dim, nx, ny, nz, self.n_nodes, self.nb
+        hs = (h[0], h[0], 0.0) if dim == 2 else tuple(h)
+        for a in range(3):
+            g.h[a] = hs[a]
+            g.inv_h[a] = (1.0 / hs[a]) if hs[a] != 0.0 else 0.0
+        self.grid = g
+        self.NY, self.NZ = _layout(dim, coupled)
+        dev = self.dev
+        n = self.n_nodes
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.cell_start = torch.zeros(n + 1, **i32)
+        self.cell_count = torch.zeros(n + 1, **i32)
+        self.scan_tmp = torch.zeros(1100, **i32)
+        self.err = torch.zeros(1, **i32)
+        self.row_ptr = torch.zeros(n + 1, **i32)
+        self.row_nnzb = torch.zeros(n + 1, **i32)
+        self.row_mask = torch.zeros(2 * n, dtype=torch.int64, device=dev)
+        self.cap = (12 if dim == 2 else 32) * n
+        self.col_idx = torch.zeros(self.cap, **i32)
+        nbv = n * self.nb
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.rhs = torch.zeros(nbv, **f64)
+        self.delta = torch.zeros(nbv, **f64)
+        self.bflag = torch.zeros(nbv, dtype=torch.uint8, device=dev)
+        self.act = torch.zeros(nbv, dtype=torch.uint8, device=dev)
+        self.freem = torch.zeros(nbv, dtype=torch.uint8, device=dev)
+        self.scal = torch.zeros(8, **f64)
+        self.part = torch.zeros(2 * max(n // 32 + 2, 2048), **f64)
+        self._alloc_vals()
+        self._ws = {}
+        self.n_particles = -1
+        self.timers = None
+        self.stats = dict(krylov_iters=[], krylov_relres=[])
+
+    # ------------------------------------------------------------------ buffers
+    def _alloc_vals(self):
+        jdt = torch.float32 if self.precision == 'mixed' else torch.float64
+        self.vals = torch.zeros(self.cap * self.nb * self.nb, dtype=jdt, device=self.dev)
+        self.mvals = None
+
+    def _grow(self, nnzb):
+        self.cap = int(nnzb * 1.15) + 16
+        self.col_idx = torch.zeros(self.cap, dtype=torch.int32, device=self.dev)
+        self._alloc_vals()
+
+    def ensure_particles(self, n):
+        if n == self.n_particles:
+            return
+        dev = self.dev
+        self.n_particles = n
+        self.sorted = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        self.pbase = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        self.ext = torch.zeros(max(n, 1), dtype=torch.uint8, device=dev)
+        self.Y = torch.zeros(max(n, 1) * self.NY, dtype=torch.float64, device=dev)
+        ddt = torch.float32 if self.precision == 'mixed' else torch.float64
+        self.D = torch.zeros(max(n, 1) * self.NY * self.NZ, dtype=ddt, device=dev)
+        self.pen_force = torch.zeros(max(n, 1), dtype=torch.float64, device=dev)
+
+    def workspace(self, nb, fp32):
+        key = (nb, fp32)
+        if key not in self._ws:
+            dt = torch.float32 if fp32 else torch.float64
+            n = self.n_nodes * nb
+            vecs = [torch.zeros(n, dtype=dt, device=self.dev) for _ in range(8)]
+            dinv = torch.zeros(self.n_nodes * nb * nb, dtype=dt, device=self.dev)
+            part = torch.zeros(2 * max(self.n_nodes // 32 + 2, 2048), dtype=torch.float64, device=self.dev)
+            sc = torch.zeros(16, dtype=torch.float64, device=self.dev)
+            ws = L.gm_krylov_ws(*[v.data_ptr() for v in vecs], dinv.data_ptr(), part.data_ptr(), sc.data_ptr())
+            self._ws[key] = (ws, vecs, dinv, part, sc)
+        return self._ws[key][0]
+
+    def _bins(self):
+        b = L.gm_bins()
+        b.n_particles = self.n_particles
+        b.cell_start = self.cell_start.data_ptr()
+        b.cell_count = self.cell_count.data_ptr()
+        b.sorted = self.sorted.data_ptr()
+        b.ext = self.ext.data_ptr()
+        b.scan_tmp = self.scan_tmp.data_ptr()
+        return b
+
+    def _pattern(self):
+        p = L.gm_pattern()
+        p.row_ptr = self.row_ptr.data_ptr()
+        p.col_idx = self.col_idx.data_ptr()
+        p.row_mask = self.row_mask.data_ptr()
+        p.nnzb = self.nnzb
+        return p
+
+    # ------------------------------------------------------------------ step phases
+    def begin_step(self, x, lp):
+        """Bin particles by stencil base node and build the block-CSR pattern."""
+        lib = L.lib()
+        self.ensure_particles(x.shape[0])
+        self.bins = self._bins()
+        L.check(lib.gm_bin_particles(L.ctypes.byref(self.grid), L.ptr(x), L.ptr(lp), L.ctypes.byref(self.bins),
+                                     L.ptr(self.pbase), L.ptr(self.err), _stream()), 'gm_bin_particles')
+        nbad = int(self.err.item())
+        if nbad:
+            raise L.GMError('%d particle stencils leave the background grid (or are not finite)' % nbad)
+        self.nnzb = 0
+        for _ in range(3):
+            pat = self._pattern()
+            rc = lib.gm_bsr_pattern(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ctypes.byref(pat),
+                                    L.ptr(self.row_nnzb), self.cap, _stream())
+            if rc == -4:
+                self._grow(pat.nnzb)
+                continue
+            L.check(rc, 'gm_bsr_pattern')
+            self.nnzb = pat.nnzb
+            break
+        self.pattern = self._pattern()
+
+    def set_boundary_from_dofmajor(self, bflag_dofmajor):
+        nb = self.nb
+        self.bflag.copy_(bflag_dofmajor.view(nb, self.n_nodes).t().reshape(-1).to(torch.uint8))
+
+    def dofmajor(self, v):
+        return v.view(self.n_nodes, self.nb).t().reshape(-1)
+
+    def from_dofmajor(self, v):
+        return v.view(self.nb, self.n_nodes).t().reshape(-1).contiguous()
+
+    def p2g(self, x, lp, threshold, pbc_flag=None, node_h_last=0.0, p_mass=0.0, p_part=None, mass=False):
+        lib = L.lib()
+        self.act.zero_()
+        mrhs = None
+        if mass:
+            if self.mvals is None or self.mvals.numel() < self.cap:
+                self.mvals = torch.zeros(self.cap, dtype=torch.float64, device=self.dev)
+            mrhs = torch.zeros(self.n_nodes, dtype=torch.float64, device=self.dev)
+        L.check(lib.gm_p2g(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ctypes.byref(self.pattern),
+                           L.ptr(x), L.ptr(lp), float(threshold), L.ptr(pbc_flag), float(node_h_last),
+                           float(p_mass), L.ptr(p_part), L.ptr(self.act), L.ptr(self.bflag),
+                           L.ptr(self.mvals) if mass else None, L.ptr(mrhs), _stream()), 'gm_p2g')
+        L.check(lib.gm_free_mask(self.n_nodes * self.nb, L.ptr(self.bflag), L.ptr(self.act), L.ptr(self.freem),
+                                 _stream()), 'gm_free_mask')
+        return mrhs
+
+    def solve_mass(self, mrhs, tol=1e-13, maxiter=20000):
+        """Consistent-mass projection M p = rhs (the reference solves it with
+        pyamg SA-BiCGStab (qc2d:288-293) or Warp Jacobi-BiCGStab (qc3d:410-412)
+        at 1e-8; here to `tol`)."""
+        lib = L.lib()
+        L.check(lib.gm_fix_empty_diag(self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx), L.ptr(self.mvals),
+                                      L.ptr(mrhs), _stream()), 'gm_fix_empty_diag')
+        xp = torch.zeros(self.n_nodes, dtype=torch.float64, device=self.dev)
+        ws = self.workspace(1, False)
+        it = L.c_int(0)
+        rr = L.c_double(0.0)
+        L.check(lib.gm_bicgstab(1, 0, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx), L.ptr(self.mvals),
+                                L.ptr(mrhs), L.ptr(xp), L.ctypes.byref(ws), tol, maxiter, 16, L.ctypes.byref(it),
+                                L.ctypes.byref(rr), _stream()), 'gm_bicgstab(mass)')
+        self.mass_iters = it.value
+        self.mass_relres = rr.value
+        return xp
+
+    def set_nodal_pressure(self, xp, u_old, u_new, require_active):
+        L.check(L.lib().gm_set_nodal_pressure(self.n_nodes, self.nb, L.ptr(self.bflag), L.ptr(self.act),
+                                              int(require_active), L.ptr(xp), L.ptr(u_old), L.ptr(u_new),
+                                              _stream()), 'gm_set_nodal_pressure')
+
+    def stage(self, args, want_tangent):
+        args.want_tangent = int(want_tangent)
+        args.tangent_fp32 = int(self.precision == 'mixed')
+        args.Y = self.Y.data_ptr()
+        args.D = self.D.data_ptr()
+        L.check(L.lib().gm_particle_stage(L.ctypes.byref(self.grid), L.ctypes.byref(args), _stream()),
+                'gm_particle_stage')
+
+    def residual(self, x, lp):
+        L.check(L.lib().gm_node_residual(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ptr(x), L.ptr(lp),
+                                         L.ptr(self.Y), L.ptr(self.freem), L.ptr(self.rhs), int(self.coupled),
+                                         _stream()), 'gm_node_residual')
+
+    def norm2(self, v, slot=0):
+        L.check(L.lib().gm_norm2(v.numel(), L.ptr(v), L.ptr(self.part), L.c_void_p(self.scal.data_ptr() + 8 * slot),
+                                 _stream()), 'gm_norm2')
+
+    def jacobian(self, x, lp):
+        fp32 = self.precision == 'mixed'
+        L.check(L.lib().gm_node_jacobian(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins),
+                                         L.ctypes.byref(self.pattern), L.ptr(x), L.ptr(lp), L.ptr(self.D),
+                                         int(fp32), L.ptr(self.freem), L.ptr(self.vals), int(fp32),
+                                         int(self.coupled), _stream()), 'gm_node_jacobian')
+
+    def solve(self, tol, maxiter, check_every=8, ir_steps=1):
+        """Solve K delta = rhs (K = -dR/du).  FP64: BiCGStab in FP64.  Mixed:
+        BiCGStab in FP32 on the FP32 Jacobian, wrapped in `ir_steps` sweeps of
+        iterative refinement whose residual rhs - K delta is formed in FP64."""
+        lib = L.lib()
+        n = self.n_nodes * self.nb
+        it = L.c_int(0)
+        rr = L.c_double(0.0)
+        if self.precision == 'fp64':
+            ws = self.workspace(self.nb, False)
+            L.check(lib.gm_bicgstab(self.nb, 0, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
+                                    L.ptr(self.vals), L.ptr(self.rhs), L.ptr(self.delta), L.ctypes.byref(ws), tol,
+                                    maxiter, check_every, L.ctypes.byref(it), L.ctypes.byref(rr), _stream()),
+                    'gm_bicgstab')
+            self.stats['krylov_iters'].append(it.value)
+            self.stats['krylov_relres'].append(rr.value)
+            return
+        ws = self.workspace(self.nb, True)
+        if not hasattr(self, '_b32'):
+            self._b32 = torch.zeros(n, dtype=torch.float32, device=self.dev)
+            self._x32 = torch.zeros(n, dtype=torch.float32, device=self.dev)
+            self._res64 = torch.zeros(n, dtype=torch.float64, device=self.dev)
+            self._ax64 = torch.zeros(n, dtype=torch.float64, device=self.dev)
+        self.delta.zero_()
+        total = 0
+        for k in range(ir_steps):
+            if k == 0:
+                src = self.rhs
+            else:
+                L.check(lib.gm_residual_refine(self.nb, 1, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
+                                               L.ptr(self.vals), L.ptr(self.rhs), L.ptr(self.delta), L.ptr(self._ax64),
+                                               L.ptr(self._res64), _stream()), 'gm_residual_refine')
+                src = self._res64
+            L.check(lib.gm_d2f(n, L.ptr(src), L.ptr(self._b32), _stream()), 'gm_d2f')
+            L.check(lib.gm_bicgstab(self.nb, 1, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
+                                    L.ptr(self.vals), L.ptr(self._b32), L.ptr(self._x32), L.ctypes.byref(ws), tol,
+                                    maxiter, check_every, L.ctypes.byref(it), L.ctypes.byref(rr), _stream()),
+                    'gm_bicgstab(fp32)')
+            total += it.value
+            L.check(lib.gm_axpy(n, 1.0, L.ptr(self._x32), 1, L.ptr(self.delta), _stream()), 'gm_axpy')
+        self.stats['krylov_iters'].append(total)
+        self.stats['krylov_relres'].append(rr.value)
+
+    def axpy(self, a, x, y):
+        L.check(L.lib().gm_axpy(x.numel(), float(a), L.ptr(x), int(x.dtype == torch.float32), L.ptr(y), _stream()),
+                'gm_axpy')
+
+    def sum_into(self, v, slot):
+        L.check(L.lib().gm_sum(v.numel(), L.ptr(v), L.ptr(self.part), L.c_void_p(self.scal.data_ptr() + 8 * slot),
+                               _stream()), 'gm_sum')
+
+    def g2p(self, x, lp, u_new, u_old, F_old, F_new, b_old, b_new, alpha_old, alpha_new, p_part):
+        L.check(L.lib().gm_g2p(L.ctypes.byref(self.grid), x.shape[0], L.ptr(x), L.ptr(lp), L.ptr(u_new),
+                               L.ptr(u_old), L.ptr(F_old), L.ptr(F_new), L.ptr(b_old), L.ptr(b_new), L.ptr(alpha_old),
+                               L.ptr(alpha_new), L.ptr(p_part), int(self.coupled), _stream()), 'gm_g2p')
+
+    def update_lp(self, F_new, lp, lp0, x, dmin, dmax):
+        dmn = (L.c_double * 3)(*(list(dmin) + [0.0] * (3 - len(dmin))))
+        dmx = (L.c_double * 3)(*(list(dmax) + [0.0] * (3 - len(dmax))))
+        L.check(L.lib().gm_update_lp(x.shape[0], self.dim, L.ptr(F_new), L.ptr(lp), float(lp0), L.ptr(x), dmn, dmx,
+                                     _stream()), 'gm_update_lp')
+
+
+def bsr_to_scipy_dofmajor(row_ptr, col_idx, vals, n_nodes, nb):
+    """Host copy of a node-block CSR matrix as a dof-major scipy CSR matrix
+    (dof = node + k*n_nodes), for tests and diagnostics."""
+    import numpy as np
+    import scipy.sparse as sp
+    rp = row_ptr.cpu().numpy().astype(np.int64)
+    ci = col_idx[:rp[-1]].cpu().numpy().astype(np.int64)
+    v = vals[:rp[-1] * nb * nb].double().cpu().numpy().reshape(-1, nb, nb)
+    rows_blk = np.repeat(np.arange(n_nodes), np.diff(rp))
+    r = (rows_blk[:, None, None] + n_nodes * np.arange(nb)[None, :, None]) * np.ones((1, 1, nb), dtype=np.int64)
+    c = (ci[:, None, None] + n_nodes * np.arange(nb)[None, None, :]) * np.ones((1, nb, 1), dtype=np.int64)
+    m = sp.coo_matrix((v.ravel(), (r.ravel(), c.ravel())), shape=(n_nodes * nb, n_nodes * nb)).tocsr()
+    m.eliminate_zeros()
+    return m
+
+
+class PhaseTimer:
+    """Per-phase HIP-event timer on the current stream."""
+
+    def __init__(self):
+        self.events = []
+
+    def mark(self, name):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.events.append((name, e))
+
+    def summary(self):
+        torch.cuda.synchronize()
+        out = {}
+        for (n0, e0), (n1, e1) in zip(self.events[:-1], self.events[1:]):
+            out[n1] = out.get(n1, 0.0) + e0.elapsed_time(e1)
+        return out

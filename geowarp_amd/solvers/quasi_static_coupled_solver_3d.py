@@ -1,0 +1,179 @@
+"""quasi_static_coupled_solver_3d: drop-in for GeoWarp
+solvers/quasi_static_coupled_solver_3d.py:56-420.
+
+3D u-p (4 dofs per node) implicit cpGIMP MPM with a penalty rigid indenter.
+``set_boundary_dofs`` is a Python callable
+``f(boundary_flag_array, n_grid_x, n_grid_y, n_nodes, dx, dy, end_x, end_y)``
+(the reference launches a Warp kernel over (nx+1, ny+1, nz+1), qc3d:400-402).
+G2P and the lp update run at the end of ``newton_iter`` as in the reference
+(qc3d:378-386).  ``indentation_force_array`` grows to the number of load steps
+(the reference fixes it at 25 entries, qc3d:162).
+"""
+import numpy as np
+import torch
+
+from ._base import _QuasiStaticBase, init_particles_rectangle
+
+TARGET_FORCE = 13567.27337196  # qc3d:52, "from E = 1e4 kPa"
+
+
+def get_indentation_loss(indentation_force, footing_incr_disp, total_steps, target=TARGET_FORCE):
+    """get_indentation_loss (qc3d:45-52)."""
+    return (indentation_force / (abs(footing_incr_disp) * total_steps) - target) ** 2
+
+
+class quasi_static_coupled_solver_3d(_QuasiStaticBase):
+    dim = 3
+    coupled = True
+
+    def __init__(self, background_grid_dict, particles_dict, iteration_dict, material_dict, porous_media_dict,
+                 load_dict, set_boundary_dofs, device=None, x_particles=None):
+        g = background_grid_dict
+        self.max_y, self.max_z = g.get('max_y'), g.get('max_z')
+        self.dy, self.dz = g['dy'], g['dz']
+        self.inv_dy, self.inv_dz = 1.0 / self.dy, 1.0 / self.dz
+        self.start_z, self.end_z = g['start_z'], g['end_z']
+        self.domain_min_z, self.domain_max_z = g['domain_min_z'], g['domain_max_z']
+        self.GIMP_lp_initial = g['dx'] / particles_dict['PPD'] / 2.0  # qc3d:100 (no -1e-6 in 3D)
+        self._common_init(background_grid_dict, particles_dict, iteration_dict, material_dict, load_dict, device,
+                          x_particles)
+        self.p_mass = self.p_vol * self.p_rho
+        self.dt = iteration_dict['dt']
+        self.phi_initial = porous_media_dict['phi_initial']
+        self.mobility_constant = porous_media_dict['mobility_constant']
+        l = load_dict
+        self.footing_region_min_x = l['footing_region_min_x']
+        self.footing_region_max_x = l['footing_region_max_x']
+        self.footing_region_min_y = l['footing_region_min_y']
+        self.footing_region_max_y = l['footing_region_max_y']
+        self.footing_initial_height = l['footing_initial_height']
+        self.footing_incr_disp = l['footing_incr_disp']
+        self.penalty_factor = l['penalty_factor']
+        self.set_boundary_dofs = set_boundary_dofs
+        dev = self.device
+        n = self.n_particles
+        self.particle_pressure_array = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.particle_traction_flag_array = torch.zeros(n, dtype=torch.bool, device=dev)
+        self.particle_pressure_boundary_flag_array = torch.zeros(n, dtype=torch.bool, device=dev)
+        self._old = torch.zeros(self.n_matrix_size, dtype=torch.float64, device=dev)
+        self._new = torch.zeros(self.n_matrix_size, dtype=torch.float64, device=dev)
+        self.youngs_modulus_diff = torch.tensor([self.youngs_modulus], dtype=torch.float64, device=dev)
+        self.indentation_force = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.indentation_force_array = torch.zeros(25, dtype=torch.float64, device=dev)
+        self.indentation_force_list = np.empty(0)
+        self.loss = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.E_grad = None
+
+    @property
+    def old_solution(self):
+        return self.engine.dofmajor(self._old)
+
+    @property
+    def new_solution(self):
+        return self.engine.dofmajor(self._new)
+
+    # ---------------------------------------------------------------- reference API
+    def reinit_everything(self):
+        """qc3d:206-221."""
+        start = (self.start_x, self.start_y, self.start_z)
+        end = (self.end_x, self.end_y, self.end_z)
+        x = init_particles_rectangle(3, start, end, self.dx, (self.n_grid_x, self.n_grid_y, self.n_grid_z), self.PPD)
+        self.x_particles.copy_(torch.tensor(x, dtype=torch.float64, device=self.device))
+        self.E_grad = None
+        self.indentation_force_list = np.empty(0)
+        self.particle_pressure_array.zero_()
+        self._initialization()
+
+    def reset_grid(self, dx, dy, dz):
+        """qc3d:223-230."""
+        self.dx, self.dy, self.dz = dx, dy, dz
+        self.inv_dx, self.inv_dy, self.inv_dz = 1.0 / dx, 1.0 / dy, 1.0 / dz
+
+    def reset_youngs_modulus(self, youngs_modulus):
+        """qc3d:232-239."""
+        self.youngs_modulus = youngs_modulus
+        self._set_lame()
+        self.youngs_modulus_diff.fill_(youngs_modulus)
+
+    def reset(self):
+        """qc3d:242-255."""
+        self._old.zero_()
+        self._new.zero_()
+        self.engine.delta.zero_()
+        self.dofStruct.boundary_flag_array.zero_()
+        self.dofStruct.activate_flag_array.zero_()
+
+    def _sync_grid(self):
+        # drivers may move the mesh between steps (3d_indentation_coupled_cpGIMP.py:202-207)
+        g = self.engine.grid
+        hs = (self.dx, self.dy, self.dz)
+        for a in range(3):
+            g.h[a] = hs[a]
+            g.inv_h[a] = 1.0 / hs[a]
+
+    def _args(self, current_step, total_steps):
+        a = self._stage_args(current_step, total_steps, self._new, self._old)
+        a.phi0 = self.phi_initial
+        a.mobility = self.mobility_constant
+        a.dt = self.dt
+        a.footing[0] = self.footing_region_min_x
+        a.footing[1] = self.footing_region_max_x
+        a.footing[2] = self.footing_region_min_y
+        a.footing[3] = self.footing_region_max_y
+        a.footing_height = self.footing_initial_height + (float(current_step) + 1.0) * self.footing_incr_disp
+        a.pen_par = self.youngs_modulus * self.penalty_factor * (self.dx / self.PPD) * (self.dy / self.PPD)
+        a.penalty = 1
+        return a
+
+    def newton_iter(self, current_step, total_steps):
+        """qc3d:258-392 (Newton loop, then G2P and the lp update)."""
+        eng = self.engine
+        step = int(current_step)
+        if step >= self.indentation_force_array.numel():
+            grown = torch.zeros(step + 1, dtype=torch.float64, device=self.device)
+            grown[:self.indentation_force_array.numel()] = self.indentation_force_array
+            self.indentation_force_array = grown
+        last = float(current_step) == float(total_steps) - 1
+
+        def after_residual():
+            eng.sum_into(eng.pen_force, 1)
+            f = eng.scal[1:2]
+            self.indentation_force.copy_(f)
+            self.indentation_force_array[step] = f[0]
+            self.loss.copy_(get_indentation_loss(f, self.footing_incr_disp, float(total_steps)))
+            if last:
+                # explicit d loss / d E at fixed u: the force is linear in E through the penalty (qc3d:323-325)
+                F = f[0]
+                S = abs(self.footing_incr_disp) * float(total_steps)
+                self.E_grad = (2.0 * (F / S - TARGET_FORCE) * F / (self.youngs_modulus * S)).reshape(1).cpu().numpy()
+
+        self._newton(current_step, total_steps, self._new, self._old, self._args(current_step, total_steps),
+                     after_residual=after_residual)
+        eng.g2p(self.x_particles, self.GIMP_lp, self._new, self._old, self.deformation_gradient_total_old,
+                self.deformation_gradient_total_new, self.left_Cauchy_Green_old, self.left_Cauchy_Green_new,
+                self.plastic_strain_old, self.plastic_strain_new, self.particle_pressure_array)
+        eng.update_lp(self.deformation_gradient_total_new, self.GIMP_lp, self.GIMP_lp_initial, self.x_particles,
+                      (self.domain_min_x, self.domain_min_y, self.domain_min_z),
+                      (self.domain_max_x, self.domain_max_y, self.domain_max_z))
+        self.indentation_force_list = np.append(self.indentation_force_list, float(self.indentation_force.item()))
+
+    def prepare_step(self, current_step, total_steps):
+        """reset, boundary callback, P2G and consistent-mass projection (qc3d:397-417)."""
+        self.reset()
+        self._sync_grid()
+        eng = self.engine
+        self.set_boundary_dofs(self.dofStruct.boundary_flag_array, self.n_grid_x, self.n_grid_y, self.n_nodes,
+                               self.dx, self.dy, self.end_x, self.end_y)
+        eng.begin_step(self.x_particles, self.GIMP_lp)
+        eng.set_boundary_from_dofmajor(self.dofStruct.boundary_flag_array)
+        pbc = self.particle_pressure_boundary_flag_array.to(torch.uint8)
+        mrhs = eng.p2g(self.x_particles, self.GIMP_lp, 1e-7, pbc, self.dz, self.p_mass, self.particle_pressure_array,
+                       mass=True)
+        self._sync_flags_out()
+        xp = eng.solve_mass(mrhs)
+        eng.set_nodal_pressure(xp, self._old, self._new, require_active=True)
+
+    def advance_one_step(self, current_step, total_steps):
+        """qc3d:396-420."""
+        self.prepare_step(current_step, total_steps)
+        self.newton_iter(current_step, total_steps)

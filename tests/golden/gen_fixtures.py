@@ -1,0 +1,50 @@
+"""Write the oracle golden fixtures used by the GPU parity tests.
+
+Run from the repository root:  python tests/golden/gen_fixtures.py [names...]
+Each fixture holds, for a reduced workload, the oracle's state after every
+load step (positions, Cauchy stress, F, b_e, pressure, hardening variable) and
+the oracle's coloured-FD Jacobian (-dR/du, reference masking) and residual at
+the first Newton iterate of the first load step.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, '..', '..')))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, '..')))
+
+import oracle.solvers as osv  # noqa: E402
+import harness as H  # noqa: E402
+
+STEPS = {'bar_nh': 3, 'bar_hencky': 2, 'bar_j2': 3, 'bar_dp': 3, 'terzaghi': 2, 'indent_nh': 2, 'indent_dp': 2}
+
+
+def gen(name):
+    t0 = time.time()
+    cfg = H.MINI[name]()
+    total = cfg['total_steps']
+    # Jacobian at the first iterate of step 0 (separate instance: prepare_step mutates state)
+    sj = H.make(osv, cfg)
+    K, r, u0 = sj.first_jacobian(0, total)
+    K = K.tocoo()
+    s = H.make(osv, cfg)
+    out = dict(jac_row=K.row.astype(np.int32), jac_col=K.col.astype(np.int32), jac_val=K.data, jac_res=r,
+               n_steps=np.array([STEPS[name]]), total_steps=np.array([total]))
+    for k in range(STEPS[name]):
+        s.advance_one_step(k, total)
+        f = H.fields(s)
+        for key, v in f.items():
+            out['s%d_%s' % (k, key)] = v
+        out['s%d_newton' % k] = np.array(s.residual_history)
+    path = os.path.join(HERE, name + '.npz')
+    np.savez_compressed(path, **out)
+    print('%-12s %6.1fs  %s  (%d KB)' % (name, time.time() - t0, path, os.path.getsize(path) // 1024))
+
+
+if __name__ == '__main__':
+    names = sys.argv[1:] or list(STEPS)
+    for n in names:
+        gen(n)

@@ -1,0 +1,87 @@
+"""Shared set-up for parity tests: build a workload for the oracle (numpy) or the
+device solvers (torch), apply its boundary callback / particle flags, run steps,
+and collect comparable fields as numpy arrays."""
+import numpy as np
+
+from geowarp_amd import problems as P
+
+# Reduced workloads used by the golden fixtures (the oracle runs them in seconds).
+MINI = {
+    'bar_nh': lambda: P.bar_nh(n_cells_x=4, n_cells_y=16, total_steps=20, tol=1e-9),
+    'bar_hencky': lambda: _with_material(P.bar_nh(n_cells_x=4, n_cells_y=16, total_steps=20, tol=1e-9),
+                                         dict(material_name='Hencky elasticity')),
+    'bar_j2': lambda: _j2_mini(),
+    'bar_dp': lambda: _dp_mini(),
+    'terzaghi': lambda: P.terzaghi(n_cells_x=2, n_cells_y=16, width=0.125, total_steps=10, dt=1.0),
+    'indent_nh': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.05, footing=0.5,
+                                    material='Neo-Hookean', tol=1e-10),
+    'indent_dp': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.01, footing=0.5,
+                                    material='Drucker-Prager', tol=1e-10, n_iter=20),
+}
+
+
+def _with_material(cfg, upd):
+    cfg['material_dict'] = dict(cfg['material_dict'], **upd)
+    return cfg
+
+
+def _j2_mini():
+    cfg = P.bar_nh(n_cells_x=4, n_cells_y=16, total_steps=5)
+    cfg['particles_dict']['p_rho'] = 2.0
+    cfg['material_dict'] = dict(youngs_modulus=1.0e4, poisson_ratio=0.3, material_name='J2', kappa=0.5, H=100.0)
+    return cfg
+
+
+def _dp_mini():
+    cfg = P.bar_nh(n_cells_x=4, n_cells_y=16, total_steps=5)
+    cfg['particles_dict']['p_rho'] = 2.0
+    cfg['material_dict'] = dict(youngs_modulus=1.0e4, poisson_ratio=0.05, material_name='Drucker-Prager',
+                                friction_angle=20.0, dilation_angle=5.0, cohesion=0.2)
+    return cfg
+
+
+def make(module, cfg, **kw):
+    """Build the solver and apply the workload's boundary / particle flags."""
+    s = P.build_solver(module, cfg, **kw)
+    is_oracle = module.__name__.startswith('oracle')
+    if cfg['solver'] == 'quasi_static_solver_2d':
+        flags = s.boundary if is_oracle else s.dofStruct.boundary_flag_array
+        cfg['set_boundary_dofs'](flags, s.n_grid_x, s.n_nodes)
+    if 'particle_flags' in cfg:
+        if is_oracle:
+            tf = getattr(s, 'traction_flag', np.zeros(s.n_particles, dtype=bool))
+            cfg['particle_flags'](s.x_particles, tf, s.pressure_boundary_flag)
+            if hasattr(s, 'traction_flag'):
+                s.traction_flag = tf
+        else:
+            cfg['particle_flags'](s.x_particles, s.particle_traction_flag_array,
+                                  s.particle_pressure_boundary_flag_array)
+    return s
+
+
+def fields(s):
+    """Comparable state of either implementation as numpy arrays."""
+    if hasattr(s, 'engine'):
+        out = dict(x=s.x_particles.cpu().numpy(), sigma=s.particle_Cauchy_stress_array.cpu().numpy(),
+                   F=s.deformation_gradient_total_old.cpu().numpy(), b=s.left_Cauchy_Green_old.cpu().numpy())
+        if hasattr(s, 'particle_pressure_array'):
+            out['p'] = s.particle_pressure_array.cpu().numpy()
+        if hasattr(s, 'indentation_force'):
+            out['force'] = np.array([float(s.indentation_force.item())])
+        out['alpha'] = s.plastic_strain_old.cpu().numpy()
+        return out
+    out = dict(x=s.x_particles.copy(), sigma=s.sigma.copy(), F=s.F_old.copy(), b=s.b_old.copy(),
+               alpha=s.alpha_old.copy())
+    if hasattr(s, 'pressure'):
+        out['p'] = s.pressure.copy()
+    if hasattr(s, 'indentation_force'):
+        out['force'] = np.array([float(s.indentation_force)])
+    return out
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.linalg.norm((a - b).ravel())
+    n = np.linalg.norm(b.ravel())
+    return d / n if n > 0 else d
