@@ -1,0 +1,224 @@
+"""Benchmark: implicit cpGIMP MPM load steps on one MI355X.
+
+Workload (default): 3D u-p indentation, soil block 1 x 1 x 0.5 m on 64 x 64 x 32
+cells (dx = 1/64), 2x2x2 particles per cell = 1,048,576 particles,
+Drucker-Prager (phi 30, c 5 kPa, psi 0, E 20 MPa, nu 0.3), 4 dofs per node,
+rigid 0.2 x 0.2 m corner indenter.  A "step" is one load step converged to
+1e-8 relative Newton residual (P2G + consistent-mass projection + Newton:
+particle stage, residual, Jacobian, block-Jacobi BiCGStab, update + G2P).
+
+The timed form is the mixed-precision solver (FP32 Jacobian and Krylov, FP64
+residual); the plain FP64 form runs the same steps from the same state and
+both are reported, with the mixed/FP64 speed ratio and field errors.
+value = Newton iterations per second of the mixed form over the timed steps.
+
+roofline: block-CSR SpMV (the Krylov kernel), timed live with HIP events on
+the solver stream over repeated launches on the last Jacobian of the timed
+region; algorithmic bytes per launch = nnzb*(b^2*s + 4) + 4*(n_nodes+1) + 2*n*s.
+cpu_baseline: the oracle (NumPy, one thread) on a bounded sample: one Newton
+iteration (residual + coloured-FD Jacobian + sparse direct solve) of the same
+workload restricted to 8 x 8 x 4 cells, scaled to the full particle count.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'Newton steps/s + adjoint grads/s, mixed vs FP64 at 1e-8 rel. residual, 1 GPU'
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2, help='timed load steps')
+    ap.add_argument('--warmup', type=int, default=1, help='untimed load steps before the timed ones')
+    ap.add_argument('--cells', type=str, default='64,64,32')
+    ap.add_argument('--material', type=str, default='Drucker-Prager')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--verbose', action='store_true')
+    return ap.parse_args()
+
+
+def build_solver(cells, precision, material, **kw):
+    import geowarp_amd.solvers as gs
+    from geowarp_amd import problems as P
+    dx = 1.0 / 64
+    cfg = P.indent3d(n_cells=cells, dx=dx, total_steps=40, depth=0.05, footing=0.2 * cells[0] / 64.0,
+                     material=material, precision=precision, tol=1e-8, n_iter=12, **kw)
+    s = P.build_solver(gs, cfg, device='cuda:0')
+    cfg['particle_flags'](s.x_particles, s.particle_traction_flag_array, s.particle_pressure_boundary_flag_array)
+    return s, cfg
+
+
+def run_steps(s, cfg, first, count, timer=None):
+    import torch
+    iters = 0
+    kry = 0
+    s.engine.stats['krylov_iters'].clear()
+    for k in range(first, first + count):
+        s.advance_one_step(k, cfg['total_steps'])
+        iters += s.newton_iterations
+    kry = sum(s.engine.stats['krylov_iters'])
+    torch.cuda.synchronize()
+    return iters, kry
+
+
+def spmv_roofline(s, reps=50):
+    import torch
+    from geowarp_amd import _lib as L
+    eng = s.engine
+    fp32 = eng.precision == 'mixed'
+    sz = 4 if fp32 else 8
+    n = eng.n_nodes * eng.nb
+    dt = torch.float32 if fp32 else torch.float64
+    x = torch.randn(n, dtype=dt, device=eng.dev)
+    y = torch.zeros(n, dtype=dt, device=eng.dev)
+    stream = L.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib = L.lib()
+
+    def launch():
+        L.check(lib.gm_bsr_spmv(eng.nb, int(fp32), int(fp32), eng.n_nodes, L.ptr(eng.row_ptr), L.ptr(eng.col_idx),
+                                L.ptr(eng.vals), L.ptr(x), L.ptr(y), stream), 'gm_bsr_spmv')
+
+    for _ in range(5):
+        launch()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        launch()
+    e1.record()
+    torch.cuda.synchronize()
+    avg_ms = e0.elapsed_time(e1) / reps
+    nb = eng.nb
+    bytes_ = eng.nnzb * (nb * nb * sz + 4) + 4 * (eng.n_nodes + 1) + 2 * n * sz
+    achieved = bytes_ / (avg_ms * 1e-3) / 1e9
+    return dict(kernel='k_spmv<%d,8,%s>' % (nb, 'float' if fp32 else 'double'), avg_us=avg_ms * 1e3,
+                bytes_per_launch=bytes_, nnzb=eng.nnzb, achieved_GBs=achieved)
+
+
+def field_errors(a, b):
+    import torch
+    out = {}
+    ua = (a.x_particles - a.x0)
+    ub = (b.x_particles - b.x0)
+    out['displacement'] = float(torch.linalg.norm(ua - ub) / torch.linalg.norm(ub))
+    out['stress'] = float(torch.linalg.norm(a.particle_Cauchy_stress_array - b.particle_Cauchy_stress_array)
+                          / torch.linalg.norm(b.particle_Cauchy_stress_array))
+    pb = torch.linalg.norm(b.particle_pressure_array)
+    out['pore_pressure'] = float(torch.linalg.norm(a.particle_pressure_array - b.particle_pressure_array) / pb) \
+        if pb > 0 else 0.0
+    return out
+
+
+def cpu_baseline():
+    """Oracle Newton iteration on an 8 x 8 x 4-cell sample of the workload."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except Exception:  # pragma: no cover
+        threadpool_limits = None
+    sys.path.insert(0, os.path.join(ROOT, 'tests'))
+    import harness as H
+    import oracle.solvers as osv
+    from geowarp_amd import problems as P
+    cells = (8, 8, 4)
+    cfg = P.indent3d(n_cells=cells, dx=1.0 / 64, total_steps=40, depth=0.05, footing=0.2 * 8 / 64.0,
+                     material='Drucker-Prager', tol=1e-8)
+    s = H.make(osv, cfg)
+    ctx = threadpool_limits(1) if threadpool_limits else None
+    t0 = time.perf_counter()
+    K, r, u0 = s.first_jacobian(0, cfg['total_steps'])
+    from scipy.sparse.linalg import spsolve
+    spsolve(K.tocsc(), r)
+    t = time.perf_counter() - t0
+    if ctx is not None:
+        ctx.unregister() if hasattr(ctx, 'unregister') else None
+    n_sample = s.n_particles
+    n_full = 64 * 64 * 32 * 8
+    t_full = t * n_full / n_sample
+    return dict(value=1.0 / t_full, unit='Newton steps/s', cores=1, kind='port',
+                sample='oracle (NumPy FP64, 1 thread) Newton iteration = residual + coloured-FD Jacobian + '
+                       'spsolve on the 8x8x4-cell sub-block (%d particles) of the same workload, %.1f s, scaled '
+                       'linearly to %d particles' % (n_sample, t, n_full))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: the path is single-device (multi-GPU sharding is a design note, '
+              'not built); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from geowarp_amd.solvers._engine import PhaseTimer
+    cells = tuple(int(c) for c in args.cells.split(','))
+    torch.cuda.set_device(0)
+
+    res = {}
+    forms = ['mixed'] if args.no_fp64 else ['fp64', 'mixed']
+    solvers = {}
+    for form in forms:
+        s, cfg = build_solver(cells, form, args.material)
+        solvers[form] = s
+        run_steps(s, cfg, 0, args.warmup)
+        s.engine.timers = PhaseTimer()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        iters, kry = run_steps(s, cfg, args.warmup, args.steps)
+        t1 = time.perf_counter()
+        phases = s.engine.timers.summary()
+        s.engine.timers = None
+        res[form] = dict(seconds=t1 - t0, newton_iters=iters, krylov_iters=kry,
+                         ms_per_step=(t1 - t0) * 1e3 / args.steps, newton_per_s=iters / (t1 - t0),
+                         phases_ms={k: round(v, 3) for k, v in phases.items()},
+                         last_residuals=['%.2e' % v for v in s.residual_history])
+        if args.verbose:
+            print(form, json.dumps(res[form]), file=sys.stderr)
+    main_form = 'mixed'
+    roof = spmv_roofline(solvers[main_form])
+    roof64 = spmv_roofline(solvers['fp64']) if 'fp64' in solvers else None
+    out = {
+        'metric': METRIC,
+        'value': round(res[main_form]['newton_per_s'], 4),
+        'unit': 'Newton steps/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(res[main_form]['ms_per_step'], 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32+f64 (FP32 Jacobian/Krylov, FP64 residual)',
+        'data': 'synthetic',
+        'config': {'workload': '3D u-p indentation %dx%dx%d cells, %d particles, %s, load steps %d-%d to 1e-8 '
+                               'relative residual' % (cells + (solvers[main_form].n_particles, args.material,
+                                                               args.warmup, args.warmup + args.steps - 1)),
+                   'nodes': solvers[main_form].n_nodes, 'dofs': solvers[main_form].n_matrix_size,
+                   'nnzb': solvers[main_form].engine.nnzb},
+        'mixed': res.get('mixed'),
+        'fp64': res.get('fp64'),
+        'roofline': {'bound': 'hbm', 'achieved': round(roof['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': round(roof['achieved_GBs'] / HBM_PEAK_GBS, 4), 'traffic': None,
+                     'kernel': roof['kernel'], 'avg_us': round(roof['avg_us'], 2),
+                     'algorithmic_bytes': roof['bytes_per_launch']},
+    }
+    if roof64:
+        out['roofline_fp64_spmv'] = {'achieved': round(roof64['achieved_GBs'], 1), 'frac': round(
+            roof64['achieved_GBs'] / HBM_PEAK_GBS, 4), 'avg_us': round(roof64['avg_us'], 2),
+            'kernel': roof64['kernel']}
+    if 'fp64' in res:
+        out['mixed_vs_fp64_speedup'] = round(res['fp64']['seconds'] / res['mixed']['seconds'], 3)
+        out['mixed_vs_fp64_field_rel_err'] = field_errors(solvers['mixed'], solvers['fp64'])
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -22,7 +22,8 @@ class gm_grid(ctypes.Structure):
 
 class gm_bins(ctypes.Structure):
     _fields_ = [('n_particles', c_int), ('cell_start', c_void_p), ('cell_count', c_void_p), ('sorted', c_void_p),
-                ('ext', c_void_p), ('scan_tmp', c_void_p)]
+                ('ext', c_void_p), ('scan_tmp', c_void_p), ('ext_s', c_void_p), ('x_s', c_void_p),
+                ('lp_s', c_void_p)]
 
 
 class gm_pattern(ctypes.Structure):
@@ -34,7 +35,7 @@ class gm_stage_args(ctypes.Structure):
                 ('x', c_void_p), ('lp', c_void_p), ('F_old', c_void_p), ('F_new', c_void_p), ('b_old', c_void_p),
                 ('b_new', c_void_p), ('sigma', c_void_p), ('alpha_old', c_void_p), ('alpha_new', c_void_p),
                 ('ext_flag', c_void_p), ('trac_flag', c_void_p), ('u_new', c_void_p), ('u_old', c_void_p),
-                ('Y', c_void_p), ('D', c_void_p), ('pen_force', c_void_p),
+                ('sorted', c_void_p), ('Y', c_void_p), ('D', c_void_p), ('pen_force', c_void_p),
                 ('mat_kind', c_int), ('dp_apex', c_int),
                 ('lam', c_double), ('mu', c_double), ('E', c_double), ('kappa', c_double), ('H', c_double),
                 ('phi', c_double), ('psi', c_double), ('coh', c_double), ('shape', c_double),
@@ -59,10 +60,10 @@ _SIGS = {
     'gm_bsr_pattern': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern), _P,
                                c_int, _P]),
     'gm_particle_stage': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_stage_args), _P]),
-    'gm_node_residual': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), _P, _P, _P, _P, _P, c_int, _P]),
+    'gm_node_residual': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), _P, _P, _P, c_int, _P]),
     'gm_node_jacobian': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern), _P,
-                                 _P, _P, c_int, _P, _P, c_int, c_int, _P]),
-    'gm_p2g': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern), _P, _P,
+                                 c_int, _P, _P, c_int, c_int, _P]),
+    'gm_p2g': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern),
                        c_double, _P, c_double, c_double, _P, _P, _P, _P, _P, _P]),
     'gm_free_mask': (c_int, [c_int, _P, _P, _P, _P]),
     'gm_set_nodal_pressure': (c_int, [c_int, c_int, _P, _P, c_int, _P, _P, _P, _P]),

@@ -129,6 +129,9 @@ GM_DEV Axis gimp_axis(double x, double lp, double h, double inv_h) {
     a.wa[0] = a0 / (4.0 * lp);
     a.wa[1] = 0.5;
     a.wa[2] = a2 / (4.0 * lp);
+    // a domain ending exactly on a cell face takes this branch with an exactly
+    // zero third node: it contributes nothing and is left out of the support
+    if (a.w[2] == 0.0 && a.g[2] == 0.0 && a.wa[2] == 0.0) a.ext = 2;
   }
   return a;
 }
@@ -413,8 +416,10 @@ GM_DEV void return_dp(const double* etr, const MatParams& m, double* e, double* 
   for (int i = 0; i < 9; ++i) dE[i] = (i % 4 == 0) ? 1.0 : 0.0;
   if (y < 1e-10) return;
   double K = m.lam + 2.0 * m.mu / 3.0;
-  bool tip = (P >= 0.0);
-  if (m.dp_apex == 1 && !tip) {
+  // a hyperbolic surface (shape factor > 0) has no apex: always return to it
+  bool smooth = (m.dp_apex == 1 && m.shape > 0.0);
+  bool tip = (P >= 0.0) && !smooth;
+  if (m.dp_apex == 1 && !tip && !smooth) {
     // the cone return would reverse the deviator (closed form, shape factor 0)
     double Sn = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
     double dl_cone = y / (2.0 * m.mu + Bf * K * Bg);

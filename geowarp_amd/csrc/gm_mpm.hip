@@ -118,6 +118,19 @@ __global__ void k_bin_scatter(int n, const int* pbase, const int* start, int* cu
   sorted[pos] = p;
 }
 
+// bin-sorted copies of the per-particle geometry (index k = position in `sorted`)
+__global__ void k_bin_finalize(int n, int dim, const int* sorted, const double* x, const double* lp,
+                               const uint8_t* ext, uint8_t* ext_s, double* x_s, double* lp_s) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int p = sorted[k];
+  ext_s[k] = ext[p];
+  for (int a = 0; a < dim; ++a) {
+    x_s[k * dim + a] = x[p * dim + a];
+    lp_s[k * dim + a] = lp[p * dim + a];
+  }
+}
+
 __global__ void k_bin_sort(int n_nodes, const int* start, int* sorted) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_nodes) return;
@@ -220,16 +233,20 @@ extern "C" int gm_bin_particles(const gm_grid* g, const double* x, const double*
     GM_LAUNCH_CHECK();
   }
   k_bin_sort<<<gm_blocks(g->n_nodes, 256), 256, 0, s>>>(g->n_nodes, b->cell_start, b->sorted);
+  if (n > 0)
+    k_bin_finalize<<<gm_blocks(n, 256), 256, 0, s>>>(n, g->dim, b->sorted, x, lp, b->ext, b->ext_s, b->x_s,
+                                                     b->lp_s);
   GM_LAUNCH_CHECK();
   return 0;
 }
 
 // ================================================================== candidate walk
 // Visit every particle whose 3^d stencil contains node I (base cell I - o,
-// o in {0,1,2}^d), in a fixed order.  f(p, o, in_support).
+// o in {0,1,2}^d), in a fixed order.  f(k, o, b, in_support) with k the
+// particle's bin-sorted position (ext_s is bin-sorted).
 template <int DIM, typename F>
 __device__ __forceinline__ void for_candidates(const gm_grid& g, const int* ci, const int* cell_start,
-                                               const int* sorted, const uint8_t* ext, F&& f) {
+                                               const uint8_t* ext_s, F&& f) {
   int ncell[3] = {g.nx, g.ny, g.nz};
   const int oz_n = (DIM == 3) ? 3 : 1;
   for (int oz = 0; oz < oz_n; ++oz) {
@@ -247,12 +264,11 @@ __device__ __forceinline__ void for_candidates(const gm_grid& g, const int* ci, 
         int bn = node_index<DIM>(g, b);
         int s = cell_start[bn], e = cell_start[bn + 1];
         for (int k = s; k < e; ++k) {
-          int p = sorted[k];
-          uint8_t ex = ext[p];
+          uint8_t ex = ext_s[k];
           bool in = true;
 #pragma unroll
           for (int a = 0; a < DIM; ++a) in = in && (o[a] < (((ex >> a) & 1) ? 3 : 2));
-          f(p, o, b, in);
+          f(k, o, b, in);
         }
       }
     }
@@ -261,8 +277,8 @@ __device__ __forceinline__ void for_candidates(const gm_grid& g, const int* ci, 
 
 // ================================================================== pattern
 template <int DIM>
-__global__ void k_pattern_count(gm_grid g, const int* cell_start, const int* sorted, const uint8_t* ext,
-                                uint64_t* mask, int* row_nnzb) {
+__global__ void k_pattern_count(gm_grid g, const int* cell_start, const uint8_t* ext_s, uint64_t* mask,
+                                int* row_nnzb) {
   int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= g.n_nodes) return;
   int ci[3] = {0, 0, 0};
@@ -271,9 +287,9 @@ __global__ void k_pattern_count(gm_grid g, const int* cell_start, const int* sor
   // centre bit
   int cb = (DIM == 2) ? (2 + 5 * 2) : (2 + 5 * 2 + 25 * 2);
   if (cb < 64) m0 |= (1ull << cb); else m1 |= (1ull << (cb - 64));
-  for_candidates<DIM>(g, ci, cell_start, sorted, ext, [&](int p, const int* o, const int* b, bool in) {
+  for_candidates<DIM>(g, ci, cell_start, ext_s, [&](int k, const int* o, const int* b, bool in) {
     if (!in) return;
-    uint8_t ex = ext[p];
+    uint8_t ex = ext_s[k];
     int e[3];
 #pragma unroll
     for (int a = 0; a < DIM; ++a) e[a] = ((ex >> a) & 1) ? 3 : 2;
@@ -315,11 +331,9 @@ extern "C" int gm_bsr_pattern(const gm_grid* g, const gm_bins* b, gm_pattern* pa
   GM_REQUIRE(g && b && pat, "gm_bsr_pattern: null argument");
   int n = g->n_nodes;
   if (g->dim == 2)
-    k_pattern_count<2><<<gm_blocks(n, 128), 128, 0, s>>>(*g, b->cell_start, b->sorted, b->ext, pat->row_mask,
-                                                         row_nnzb);
+    k_pattern_count<2><<<gm_blocks(n, 128), 128, 0, s>>>(*g, b->cell_start, b->ext_s, pat->row_mask, row_nnzb);
   else
-    k_pattern_count<3><<<gm_blocks(n, 128), 128, 0, s>>>(*g, b->cell_start, b->sorted, b->ext, pat->row_mask,
-                                                         row_nnzb);
+    k_pattern_count<3><<<gm_blocks(n, 128), 128, 0, s>>>(*g, b->cell_start, b->ext_s, pat->row_mask, row_nnzb);
   GM_LAUNCH_CHECK();
   int rc = exclusive_scan(row_nnzb, pat->row_ptr, n, b->scan_tmp, s);
   if (rc) return rc;
@@ -343,8 +357,9 @@ extern "C" int gm_bsr_pattern(const gm_grid* g, const gm_bins* b, gm_pattern* pa
 template <int DIM, bool COUPLED, typename TD>
 __global__ void __launch_bounds__(128) k_particle_stage(gm_grid g, gm_stage_args a) {
   typedef Lay<DIM, COUPLED> L;
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n_particles) return;
+  int kq = blockIdx.x * blockDim.x + threadIdx.x;  // bin-sorted position: Y/D row
+  if (kq >= a.n_particles) return;
+  int p = a.sorted ? a.sorted[kq] : kq;               // particle id: state arrays
   Axis ax[DIM];
   load_axes<DIM>(g, a.x, a.lp, p, ax);
   double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -521,11 +536,11 @@ __global__ void __launch_bounds__(128) k_particle_stage(gm_grid g, gm_stage_args
     Y[L::Y_S] = tau_s * cpp * V;
   }
 #pragma unroll
-  for (int y = 0; y < L::NY; ++y) a.Y[(long)p * L::NY + y] = Y[y];
+  for (int y = 0; y < L::NY; ++y) a.Y[(long)kq * L::NY + y] = Y[y];
   if (!a.want_tangent) return;
 
   // ---- tangent D = dY/dz, written column by column (every entry exactly once)
-  TD* D = (TD*)a.D + (long)p * L::NY * L::NZ;
+  TD* D = (TD*)a.D + (long)kq * L::NY * L::NZ;
   auto put = [&](int y, int z, double v) { D[y * L::NZ + z] = (TD)v; };
   // M = tau - p J I; P = -V0 A^T M
   M3 Mm = tau;
@@ -690,10 +705,9 @@ extern "C" int gm_particle_stage(const gm_grid* g, const gm_stage_args* a, hipSt
 
 // ================================================================== node residual
 template <int DIM, bool COUPLED>
-__global__ void __launch_bounds__(128) k_node_residual(gm_grid g, const double* x, const double* lp,
-                                                       const int* cell_start, const int* sorted,
-                                                       const uint8_t* ext, const double* Y, const uint8_t* freem,
-                                                       double* rhs) {
+__global__ void __launch_bounds__(128) k_node_residual(gm_grid g, const double* x_s, const double* lp_s,
+                                                       const int* cell_start, const uint8_t* ext_s, const double* Y,
+                                                       const uint8_t* freem, double* rhs) {
   typedef Lay<DIM, COUPLED> L;
   int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= g.n_nodes) return;
@@ -702,13 +716,13 @@ __global__ void __launch_bounds__(128) k_node_residual(gm_grid g, const double* 
   double r[L::NB];
 #pragma unroll
   for (int k = 0; k < L::NB; ++k) r[k] = 0.0;
-  for_candidates<DIM>(g, ci, cell_start, sorted, ext, [&](int p, const int* o, const int* b, bool in) {
+  for_candidates<DIM>(g, ci, cell_start, ext_s, [&](int k, const int* o, const int* b, bool in) {
     if (!in) return;
     Axis ax[DIM];
-    load_axes<DIM>(g, x, lp, p, ax);
+    load_axes<DIM>(g, x_s, lp_s, k, ax);
     double w, G[3], wa;
     node_geom<DIM>(ax, o, w, G, wa);
-    const double* yp = Y + (long)p * L::NY;
+    const double* yp = Y + (long)k * L::NY;
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
       double s = 0.0;
@@ -728,17 +742,17 @@ __global__ void __launch_bounds__(128) k_node_residual(gm_grid g, const double* 
   for (int k = 0; k < L::NB; ++k) rhs[I * L::NB + k] = freem[I * L::NB + k] ? r[k] : 0.0;
 }
 
-extern "C" int gm_node_residual(const gm_grid* g, const gm_bins* b, const double* x, const double* lp,
-                                const double* Y, const uint8_t* freem, double* rhs, int coupled, hipStream_t s) {
+extern "C" int gm_node_residual(const gm_grid* g, const gm_bins* b, const double* Y, const uint8_t* freem,
+                                double* rhs, int coupled, hipStream_t s) {
   GM_REQUIRE(g && b, "gm_node_residual: null argument");
   int n = g->n_nodes;
   int blk = gm_blocks(n, 128);
   if (g->dim == 2 && !coupled)
-    k_node_residual<2, false><<<blk, 128, 0, s>>>(*g, x, lp, b->cell_start, b->sorted, b->ext, Y, freem, rhs);
+    k_node_residual<2, false><<<blk, 128, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, b->ext_s, Y, freem, rhs);
   else if (g->dim == 2)
-    k_node_residual<2, true><<<blk, 128, 0, s>>>(*g, x, lp, b->cell_start, b->sorted, b->ext, Y, freem, rhs);
+    k_node_residual<2, true><<<blk, 128, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, b->ext_s, Y, freem, rhs);
   else if (coupled)
-    k_node_residual<3, true><<<blk, 128, 0, s>>>(*g, x, lp, b->cell_start, b->sorted, b->ext, Y, freem, rhs);
+    k_node_residual<3, true><<<blk, 128, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, b->ext_s, Y, freem, rhs);
   else {
     gm::set_error("gm_node_residual: 3D single-phase is not a supported form");
     return -1;
@@ -750,9 +764,9 @@ extern "C" int gm_node_residual(const gm_grid* g, const gm_bins* b, const double
 // ================================================================== node Jacobian
 // One 64-lane wavefront per block row.  Lane l owns compressed slots l and l+64.
 template <int DIM, bool COUPLED, typename TD, typename TV>
-__global__ void __launch_bounds__(64) k_node_jacobian(gm_grid g, const double* x, const double* lp,
-                                                      const int* cell_start, const int* sorted, const uint8_t* ext,
-                                                      const TD* D, const uint8_t* freem, const int* row_ptr,
+__global__ void __launch_bounds__(64) k_node_jacobian(gm_grid g, const double* x_s, const double* lp_s,
+                                                      const int* cell_start, const uint8_t* ext_s, const TD* D,
+                                                      const uint8_t* freem, const int* row_ptr,
                                                       const uint64_t* row_mask, TV* vals) {
   typedef Lay<DIM, COUPLED> L;
   const int NB = L::NB, NZ = L::NZ, NY = L::NY;
@@ -794,13 +808,13 @@ __global__ void __launch_bounds__(64) k_node_jacobian(gm_grid g, const double* x
 #pragma unroll
     for (int k = 0; k < NB * NB; ++k) acc[s2][k] = 0.0;
 
-  for_candidates<DIM>(g, ci, cell_start, sorted, ext, [&](int p, const int* o, const int* b, bool in) {
+  for_candidates<DIM>(g, ci, cell_start, ext_s, [&](int k, const int* o, const int* b, bool in) {
     if (!in) return;  // uniform across the wavefront
     Axis ax[DIM];
-    load_axes<DIM>(g, x, lp, p, ax);
+    load_axes<DIM>(g, x_s, lp_s, k, ax);
     double wI, GI[3], waI;
     node_geom<DIM>(ax, o, wI, GI, waI);
-    const TD* Dp = D + (long)p * NY * NZ;
+    const TD* Dp = D + (long)k * NY * NZ;
     // E = L_I D_p  (NB x NZ), one entry per lane
     if (lane < NB * NZ) {
       int r = lane / NZ, z = lane % NZ;
@@ -826,7 +840,7 @@ __global__ void __launch_bounds__(64) k_node_jacobian(gm_grid g, const double* x
 #pragma unroll
       for (int aa = 0; aa < DIM; ++aa) {
         oJ[aa] = o[aa] + myoff[s2][aa];  // offset of J inside this particle's stencil
-        int e = ((ext[p] >> aa) & 1) ? 3 : 2;
+        int e = ((ext_s[k] >> aa) & 1) ? 3 : 2;
         inJ = inJ && (oJ[aa] >= 0) && (oJ[aa] < e);
       }
       if (!inJ) continue;
@@ -874,14 +888,14 @@ __global__ void __launch_bounds__(64) k_node_jacobian(gm_grid g, const double* x
   }
 }
 
-extern "C" int gm_node_jacobian(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, const double* x,
-                                const double* lp, const void* D, int tangent_fp32, const uint8_t* freem, void* vals,
-                                int vals_fp32, int coupled, hipStream_t s) {
+extern "C" int gm_node_jacobian(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, const void* D,
+                                int tangent_fp32, const uint8_t* freem, void* vals, int vals_fp32, int coupled,
+                                hipStream_t s) {
   GM_REQUIRE(g && b && pat, "gm_node_jacobian: null argument");
   int n = g->n_nodes;
-#define GM_JAC(DIM, CP, TD, TV)                                                                               \
-  k_node_jacobian<DIM, CP, TD, TV><<<n, 64, 0, s>>>(*g, x, lp, b->cell_start, b->sorted, b->ext, (const TD*)D, \
-                                                    freem, pat->row_ptr, pat->row_mask, (TV*)vals)
+#define GM_JAC(DIM, CP, TD, TV)                                                                           \
+  k_node_jacobian<DIM, CP, TD, TV><<<n, 64, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, b->ext_s,          \
+                                                    (const TD*)D, freem, pat->row_ptr, pat->row_mask, (TV*)vals)
 #define GM_JAC_T(DIM, CP)                                  \
   do {                                                     \
     if (tangent_fp32 && vals_fp32) GM_JAC(DIM, CP, float, float);   \
@@ -909,8 +923,8 @@ extern "C" int gm_node_jacobian(const gm_grid* g, const gm_bins* b, const gm_pat
 //  mvals:   consistent mass in the node pattern (b = 1), may be null
 //  mrhs:    consistent-mass rhs, may be null
 template <int DIM>
-__global__ void k_p2g(gm_grid g, int nb, const double* x, const double* lp, const int* cell_start,
-                      const int* sorted, const uint8_t* ext, double threshold, const uint8_t* pbc_flag,
+__global__ void k_p2g(gm_grid g, int nb, const double* x_s, const double* lp_s, const int* cell_start,
+                      const int* sorted, const uint8_t* ext_s, double threshold, const uint8_t* pbc_flag,
                       double node_h_last, double p_mass, const double* p_part, const int* row_ptr,
                       const uint64_t* row_mask, uint8_t* act, uint8_t* bflag, double* mvals, double* mrhs) {
   int I = blockIdx.x * blockDim.x + threadIdx.x;
@@ -921,21 +935,22 @@ __global__ void k_p2g(gm_grid g, int nb, const double* x, const double* lp, cons
   double rhs = 0.0;
   int rp = row_ptr ? row_ptr[I] : 0;
   uint64_t m0 = row_mask ? row_mask[2 * I] : 0, m1 = row_mask ? row_mask[2 * I + 1] : 0;
-  for_candidates<DIM>(g, ci, cell_start, sorted, ext, [&](int p, const int* o, const int* b, bool in) {
+  for_candidates<DIM>(g, ci, cell_start, ext_s, [&](int k, const int* o, const int* b, bool in) {
+    int p = sorted[k];
     // pressure boundary: 2D loops over the support, 3D over all 27 nodes (mpm_utils.py:225-235, 349-352)
     if (pbc_flag && pbc_flag[p] && (in || DIM == 3)) {
       double nc = (double)ci[DIM - 1] * node_h_last;
-      if (nc >= x[p * DIM + DIM - 1]) pb = true;
+      if (nc >= x_s[k * DIM + DIM - 1]) pb = true;
     }
     if (!in) return;
     Axis ax[DIM];
-    load_axes<DIM>(g, x, lp, p, ax);
+    load_axes<DIM>(g, x_s, lp_s, k, ax);
     double w, G[3], wa;
     node_geom<DIM>(ax, o, w, G, wa);
     if (threshold < 0.0 || w > threshold) a_on = true;
     if (mvals || mrhs) {
       double wsum = 0.0;
-      uint8_t ex = ext[p];
+      uint8_t ex = ext_s[k];
       int e[3];
 #pragma unroll
       for (int aa = 0; aa < DIM; ++aa) e[aa] = ((ex >> aa) & 1) ? 3 : 2;
@@ -964,10 +979,9 @@ __global__ void k_p2g(gm_grid g, int nb, const double* x, const double* lp, cons
   if (mrhs) mrhs[I] = rhs;
 }
 
-extern "C" int gm_p2g(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, const double* x, const double* lp,
-                      double threshold, const uint8_t* pbc_flag, double node_h_last, double p_mass,
-                      const double* p_part, uint8_t* act, uint8_t* bflag, double* mvals, double* mrhs,
-                      hipStream_t s) {
+extern "C" int gm_p2g(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, double threshold,
+                      const uint8_t* pbc_flag, double node_h_last, double p_mass, const double* p_part, uint8_t* act,
+                      uint8_t* bflag, double* mvals, double* mrhs, hipStream_t s) {
   GM_REQUIRE(g && b, "gm_p2g: null argument");
   int n = g->n_nodes;
   if (mvals) {
@@ -977,11 +991,11 @@ extern "C" int gm_p2g(const gm_grid* g, const gm_bins* b, const gm_pattern* pat,
   const int* rp = pat ? pat->row_ptr : nullptr;
   const uint64_t* rm = pat ? pat->row_mask : nullptr;
   if (g->dim == 2)
-    k_p2g<2><<<gm_blocks(n, 128), 128, 0, s>>>(*g, g->nb, x, lp, b->cell_start, b->sorted, b->ext, threshold,
+    k_p2g<2><<<gm_blocks(n, 128), 128, 0, s>>>(*g, g->nb, b->x_s, b->lp_s, b->cell_start, b->sorted, b->ext_s, threshold,
                                                pbc_flag, node_h_last, p_mass, p_part, rp, rm, act, bflag, mvals,
                                                mrhs);
   else
-    k_p2g<3><<<gm_blocks(n, 128), 128, 0, s>>>(*g, g->nb, x, lp, b->cell_start, b->sorted, b->ext, threshold,
+    k_p2g<3><<<gm_blocks(n, 128), 128, 0, s>>>(*g, g->nb, b->x_s, b->lp_s, b->cell_start, b->sorted, b->ext_s, threshold,
                                                pbc_flag, node_h_last, p_mass, p_part, rp, rm, act, bflag, mvals,
                                                mrhs);
   GM_LAUNCH_CHECK();

@@ -25,8 +25,12 @@ typedef struct gm_bins {
   int* cell_start;   /* n_nodes + 1 */
   int* cell_count;   /* n_nodes + 1 (scratch) */
   int* sorted;       /* n_particles: particle ids ordered by base node, then id */
-  uint8_t* ext;      /* n_particles: bit a set => 3 nodes along axis a, else 2 */
-  int* scan_tmp;     /* >= 1024 ints of scratch */
+  uint8_t* ext;      /* n_particles, particle order: bit a set => 3 nodes along axis a, else 2 */
+  int* scan_tmp;     /* >= 1025 ints of scratch */
+  /* bin-sorted copies (index k = position in `sorted`), written by gm_bin_particles */
+  uint8_t* ext_s;    /* n_particles */
+  double* x_s;       /* n_particles x dim */
+  double* lp_s;      /* n_particles x dim */
 } gm_bins;
 
 /* Per-step block-CSR pattern over nodes (5^dim window around each node). */
@@ -53,8 +57,9 @@ typedef struct gm_stage_args {
   const uint8_t* trac_flag;  /* n_p traction flags, may be null */
   const double* u_new;  /* single phase: increment; u-p: new solution (node-interleaved) */
   const double* u_old;  /* u-p: old solution (node-interleaved) */
-  double* Y;            /* n_p x nY particle "stress vector" */
-  void* D;              /* n_p x nY x nZ tangent (double or float) */
+  const int* sorted;    /* bin order (gm_bins.sorted) or null; Y and D rows follow it */
+  double* Y;            /* n_p x nY particle "stress vector", bin-sorted rows */
+  void* D;              /* n_p x nY x nZ tangent (double or float), bin-sorted rows */
   double* pen_force;    /* n_p per-particle indentation force (3D), may be null */
   /* material */
   int mat_kind; int dp_apex;

@@ -25,6 +25,10 @@ Configurations (the sizes in brackets are the full benchmark ones):
               c 5 kPa, psi 0, E 20 MPa, nu 0.3, k/mu_f 1e-6, rigid 0.2 x 0.2 m
               indenter at the corner, 0.05 m in 40 steps, penalty factor 1000.
               Background grid padded by 1 cell below x/y and 3 cells above.
+              The particle-domain box (lp clipping, gimp.py:489-505) ends at the
+              initial soil surface, so heaving particle domains do not reach
+              into the empty nodes above it (clip_top=False keeps the grid top,
+              as the reference driver does).
 """
 import numpy as np
 
@@ -180,14 +184,16 @@ def make_indent_bc():
 
 
 def indent3d(n_cells=(64, 64, 32), dx=1.0 / 64, total_steps=40, depth=0.05, footing=0.2, material='Drucker-Prager',
-             E=2.0e4, precision='fp64', tol=1e-8, n_iter=10, mobility=1e-6, dt=1.0):
+             E=2.0e4, precision='fp64', tol=1e-8, n_iter=10, mobility=1e-6, dt=1.0, gravity=0.0,
+             dp_apex='reference', shape_factor=0.0, clip_top=True):
     ncx, ncy, ncz = n_cells
     nx, ny, nz = ncx + 4, ncy + 4, ncz + 3
     sx, sy, sz = dx, dx, 0.0
     ex, ey, ez = sx + ncx * dx, sy + ncy * dx, ncz * dx
     g = dict(n_grid_x=nx, n_grid_y=ny, n_grid_z=nz, max_x=nx * dx, max_y=ny * dx, max_z=nz * dx, dx=dx, dy=dx,
              dz=dx, start_x=sx, end_x=ex, start_y=sy, end_y=ey, start_z=sz, end_z=ez, domain_min_x=sx,
-             domain_max_x=ex, domain_min_y=sy, domain_max_y=ey, domain_min_z=sz, domain_max_z=nz * dx)
+             domain_max_x=ex, domain_min_y=sy, domain_max_y=ey, domain_min_z=sz,
+             domain_max_z=ez if clip_top else nz * dx)
     PPD = 2
     npart = ncx * ncy * ncz * PPD ** 3
 
@@ -201,9 +207,9 @@ def indent3d(n_cells=(64, 64, 32), dx=1.0 / 64, total_steps=40, depth=0.05, foot
         particles_dict=dict(n_particles=npart, PPD=PPD, p_vol=(dx / PPD) ** 3, p_rho=2.0),
         iteration_dict=dict(n_iter=n_iter, tol=tol, tol_mode='relative', dt=dt, precision=precision),
         material_dict=dict(youngs_modulus=E, poisson_ratio=0.3, material_name=material, friction_angle=30.0,
-                           dilation_angle=0.0, cohesion=5.0, dp_apex='reference'),
+                           dilation_angle=0.0, cohesion=5.0, dp_apex=dp_apex, shape_factor=shape_factor),
         porous_media_dict=dict(phi_initial=0.5, mobility_constant=mobility),
-        load_dict=dict(gravity_mag=0.0, traction_value_x=0.0, traction_value_y=0.0, traction_value_z=0.0,
+        load_dict=dict(gravity_mag=gravity, traction_value_x=0.0, traction_value_y=0.0, traction_value_z=0.0,
                        point_load_value_x=0.0, point_load_value_y=0.0, point_load_value_z=0.0,
                        footing_region_min_x=sx, footing_region_max_x=sx + footing, footing_region_min_y=sy,
                        footing_region_max_y=sy + footing, footing_initial_height=ez - dx / PPD,

@@ -196,15 +196,26 @@ class _QuasiStaticBase:
         self.residual_history = []
         r0 = None
         it = 0
+        tm = eng.timers
         for it in range(self.n_iter):
+            if tm is not None:
+                tm.mark('start')
             eng.stage(args, want_tangent=True)
+            if tm is not None:
+                tm.mark('particle_stage')
             eng.residual(x, lp)
             eng.norm2(eng.rhs, 0)
             if after_residual is not None:
                 after_residual()
+            if tm is not None:
+                tm.mark('node_residual')
             eng.jacobian(x, lp)
+            if tm is not None:
+                tm.mark('node_jacobian')
             eng.solve(self.linear_tol, self.linear_maxiter, ir_steps=self.ir_steps)
             eng.axpy(1.0, eng.delta, u_new)
+            if tm is not None:
+                tm.mark('krylov')
             nr = math.sqrt(max(eng.scal[0].item(), 0.0))
             self.residual_history.append(nr)
             if self.verbose:

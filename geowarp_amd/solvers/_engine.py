@@ -13,6 +13,7 @@ Nodal vectors are node-interleaved internally (value k of node n at n*nb+k);
 the solver classes expose the reference's dof-major views.
 """
 import math
+import os
 import time
 
 import torch
@@ -72,7 +73,7 @@ class Engine:
         self.row_ptr = torch.zeros(n + 1, **i32)
         self.row_nnzb = torch.zeros(n + 1, **i32)
         self.row_mask = torch.zeros(2 * n, dtype=torch.int64, device=dev)
-        self.cap = (12 if dim == 2 else 32) * n
+        self.cap = int(os.environ.get('GM_INITIAL_BLOCKS_PER_NODE', 12 if dim == 2 else 32)) * n
         self.col_idx = torch.zeros(self.cap, **i32)
         nbv = n * self.nb
         f64 = dict(dtype=torch.float64, device=dev)
@@ -108,6 +109,9 @@ class Engine:
         self.sorted = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
         self.pbase = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
         self.ext = torch.zeros(max(n, 1), dtype=torch.uint8, device=dev)
+        self.ext_s = torch.zeros(max(n, 1), dtype=torch.uint8, device=dev)
+        self.x_s = torch.zeros(max(n, 1) * self.dim, dtype=torch.float64, device=dev)
+        self.lp_s = torch.zeros(max(n, 1) * self.dim, dtype=torch.float64, device=dev)
         self.Y = torch.zeros(max(n, 1) * self.NY, dtype=torch.float64, device=dev)
         ddt = torch.float32 if self.precision == 'mixed' else torch.float64
         self.D = torch.zeros(max(n, 1) * self.NY * self.NZ, dtype=ddt, device=dev)
@@ -134,6 +138,9 @@ class Engine:
         b.sorted = self.sorted.data_ptr()
         b.ext = self.ext.data_ptr()
         b.scan_tmp = self.scan_tmp.data_ptr()
+        b.ext_s = self.ext_s.data_ptr()
+        b.x_s = self.x_s.data_ptr()
+        b.lp_s = self.lp_s.data_ptr()
         return b
 
     def _pattern(self):
@@ -187,7 +194,7 @@ class Engine:
                 self.mvals = torch.zeros(self.cap, dtype=torch.float64, device=self.dev)
             mrhs = torch.zeros(self.n_nodes, dtype=torch.float64, device=self.dev)
         L.check(lib.gm_p2g(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ctypes.byref(self.pattern),
-                           L.ptr(x), L.ptr(lp), float(threshold), L.ptr(pbc_flag), float(node_h_last),
+                           float(threshold), L.ptr(pbc_flag), float(node_h_last),
                            float(p_mass), L.ptr(p_part), L.ptr(self.act), L.ptr(self.bflag),
                            L.ptr(self.mvals) if mass else None, L.ptr(mrhs), _stream()), 'gm_p2g')
         L.check(lib.gm_free_mask(self.n_nodes * self.nb, L.ptr(self.bflag), L.ptr(self.act), L.ptr(self.freem),
@@ -220,15 +227,16 @@ class Engine:
     def stage(self, args, want_tangent):
         args.want_tangent = int(want_tangent)
         args.tangent_fp32 = int(self.precision == 'mixed')
+        args.sorted = self.sorted.data_ptr()
         args.Y = self.Y.data_ptr()
         args.D = self.D.data_ptr()
         L.check(L.lib().gm_particle_stage(L.ctypes.byref(self.grid), L.ctypes.byref(args), _stream()),
                 'gm_particle_stage')
 
     def residual(self, x, lp):
-        L.check(L.lib().gm_node_residual(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ptr(x), L.ptr(lp),
-                                         L.ptr(self.Y), L.ptr(self.freem), L.ptr(self.rhs), int(self.coupled),
-                                         _stream()), 'gm_node_residual')
+        L.check(L.lib().gm_node_residual(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ptr(self.Y),
+                                         L.ptr(self.freem), L.ptr(self.rhs), int(self.coupled), _stream()),
+                'gm_node_residual')
 
     def norm2(self, v, slot=0):
         L.check(L.lib().gm_norm2(v.numel(), L.ptr(v), L.ptr(self.part), L.c_void_p(self.scal.data_ptr() + 8 * slot),
@@ -237,9 +245,9 @@ class Engine:
     def jacobian(self, x, lp):
         fp32 = self.precision == 'mixed'
         L.check(L.lib().gm_node_jacobian(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins),
-                                         L.ctypes.byref(self.pattern), L.ptr(x), L.ptr(lp), L.ptr(self.D),
-                                         int(fp32), L.ptr(self.freem), L.ptr(self.vals), int(fp32),
-                                         int(self.coupled), _stream()), 'gm_node_jacobian')
+                                         L.ctypes.byref(self.pattern), L.ptr(self.D), int(fp32), L.ptr(self.freem),
+                                         L.ptr(self.vals), int(fp32), int(self.coupled), _stream()),
+                'gm_node_jacobian')
 
     def solve(self, tol, maxiter, check_every=8, ir_steps=1):
         """Solve K delta = rhs (K = -dR/du).  FP64: BiCGStab in FP64.  Mixed:
@@ -321,7 +329,9 @@ def bsr_to_scipy_dofmajor(row_ptr, col_idx, vals, n_nodes, nb):
 
 
 class PhaseTimer:
-    """Per-phase HIP-event timer on the current stream."""
+    """Per-phase HIP-event timer on the current stream: mark('start') opens an
+    interval, every later mark(name) closes the interval since the previous mark
+    and charges it to `name` (milliseconds)."""
 
     def __init__(self):
         self.events = []
@@ -335,5 +345,7 @@ class PhaseTimer:
         torch.cuda.synchronize()
         out = {}
         for (n0, e0), (n1, e1) in zip(self.events[:-1], self.events[1:]):
+            if n1 == 'start':
+                continue
             out[n1] = out.get(n1, 0.0) + e0.elapsed_time(e1)
         return out

@@ -1,0 +1,156 @@
+/* geowarp_amd.h -- C ABI of libgeowarp_amd.so (MI355X / gfx950).
+ *
+ * Drop-in replacement for the numerical core of GeoWarp's implicit cpGIMP MPM
+ * load-step Newton loop.  The reference implements each phase as a Warp
+ * kernel (@wp.kernel, compiled at run time) launched from the solver classes;
+ * each entry point below cites the reference interface it replaces.
+ *
+ * Conventions
+ *   - every pointer is a device pointer unless stated otherwise; arrays are
+ *     row-major; particle tensors are (n_particles, ...) FP64;
+ *   - nodal vectors are node-interleaved: value k of node n at n*nb + k
+ *     (the Python layer converts to and from the reference's dof-major order,
+ *     dof = node + k*n_nodes);
+ *   - every call is ordered on `stream` (a hipStream_t) and returns 0 on
+ *     success or a negative code; gm_last_error() then describes the failure;
+ *   - the library allocates no device memory: buffers and workspaces are
+ *     passed in (the Python layer allocates them with torch);
+ *   - calls that return a host value (gm_bsr_pattern's block count,
+ *     gm_bicgstab's iteration count) synchronise `stream`.
+ *   The argument structs gm_grid, gm_bins, gm_pattern, gm_stage_args and
+ *   gm_krylov_ws are defined in geowarp_amd/csrc/gm_types.h.
+ */
+#ifndef GEOWARP_AMD_H
+#define GEOWARP_AMD_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../geowarp_amd/csrc/gm_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library version and the message of the last failed call (thread-local). */
+int gm_version(void);
+const char* gm_last_error(void);
+
+/* Exclusive prefix sum of n ints (out has n+1 entries; out[n] = total).
+   tmp >= 1025 ints.  Helper of the binning and pattern passes (no reference
+   counterpart; the reference sorts COO triplets in wps.bsr_set_from_triplets,
+   qs2d:202). */
+int gm_exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t stream);
+
+/* Sort particle ids by the node index of their cpGIMP stencil base
+   (gimp.py:33-49 floors), record each particle's per-axis extent and write
+   bin-sorted copies of x, lp and the extents (gm_bins.x_s, lp_s, ext_s) that
+   the node-gather kernels read.
+   pbase: n_particles ints of scratch; err: one int, set to the number of
+   particles whose stencil leaves the grid (the reference does not check).
+   Replaces the per-particle index arithmetic repeated in every kernel of
+   mpm/mpm_utils.py (e.g. :103-113, :1088-1100). */
+int gm_bin_particles(const gm_grid* g, const double* x, const double* lp, gm_bins* bins, int* pbase, int* err,
+                     hipStream_t stream);
+
+/* Per-load-step node block-CSR pattern: block (I,J) exists when a particle's
+   support holds both nodes.  row_nnzb: n_nodes+1 ints of scratch; returns -4
+   with pat->nnzb set when max_nnzb is too small.  Replaces the fixed-slot COO
+   layout of sparse_differentiation.py:213-401 (5^d slots per row and dof
+   type) and wps.bsr_set_from_triplets (qs2d:202, qc3d:336). */
+int gm_bsr_pattern(const gm_grid* g, const gm_bins* bins, gm_pattern* pat, int* row_nnzb, int max_nnzb,
+                   hipStream_t stream);
+
+/* Per-particle stage of one residual evaluation: gather the nodal increment
+   (and pressures), update F, b_e, Cauchy stress, hardening variable; write the
+   particle stress vector Y and (want_tangent) its analytic tangent D, both in
+   bin-sorted row order when a->sorted is given.
+   Replaces the particle part of assemble_residual_2d_hencky / _J2 /
+   assemble_residual_coupled_2d_neohookean / _3d_neohookean
+   (mpm_utils.py:416-1294) and the coloured reverse sweeps that differentiate
+   them (qs2d:171-190, qc2d:197-224, qc3d:285-321); materials from
+   plasticity.py:8-37 and triaxial_DruckerPrager.py:308-412. */
+int gm_particle_stage(const gm_grid* g, const gm_stage_args* a, hipStream_t stream);
+
+/* Node-gather residual r_I = sum_p L(geom_I) Y_p, zero at fixed/inactive dofs
+   (freem = !boundary && active, node-interleaved).  Replaces the atomic
+   scatter of the assemble_residual_* kernels (e.g. mpm_utils.py:575-596,
+   1252-1294). */
+int gm_node_residual(const gm_grid* g, const gm_bins* bins, const double* Y, const uint8_t* freem, double* rhs,
+                     int coupled, hipStream_t stream);
+
+/* Node-gather Jacobian into the pattern: K = -dR/du between free & active
+   dofs, identity on the others (reference sign and masking:
+   sparse_differentiation.py:253-262, mpm_utils.py:1299-1341).  D and vals may
+   be FP32 (tangent_fp32 / vals_fp32) for the mixed-precision form. */
+int gm_node_jacobian(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, const void* D, int tangent_fp32,
+                     const uint8_t* freem, void* vals, int vals_fp32, int coupled, hipStream_t stream);
+
+/* P2G: activation flags (threshold < 0: every support node, P2G_2d
+   mpm_utils.py:83-136; else w > threshold, P2G_coupled_2d/3d :209-223,
+   :341-347), pressure-Dirichlet flags above flagged particles (:225-235,
+   :349-352; node_h_last = dy in 2D, dz in 3D), and optionally the consistent
+   mass matrix in the pattern (b = 1) and its rhs (:237-257, :354-383). */
+int gm_p2g(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, double threshold, const uint8_t* pbc_flag,
+           double node_h_last, double p_mass, const double* p_part, uint8_t* act, uint8_t* bflag, double* mvals,
+           double* mrhs, hipStream_t stream);
+
+/* freem[i] = !bflag[i] && act[i] over n dofs. */
+int gm_free_mask(int n, const uint8_t* bflag, const uint8_t* act, uint8_t* freem, hipStream_t stream);
+
+/* P2G_set_nodal_solution / _3d (mpm_utils.py:387-414). */
+int gm_set_nodal_pressure(int n_nodes, int nb, const uint8_t* bflag, const uint8_t* act, int require_active,
+                          const double* xp, double* old_solution, double* new_solution, hipStream_t stream);
+
+/* G2P_2d / G2P_coupled_2d / G2P_coupled_3d (mpm_utils.py:1351-1590): move
+   particles, commit F, b_e and the hardening variable, interpolate pressure. */
+int gm_g2p(const gm_grid* g, int n, double* x, const double* lp, const double* u_new, const double* u_old,
+           double* F_old, const double* F_new, double* b_old, const double* b_new, double* alpha_old,
+           const double* alpha_new, double* p_part, int coupled, hipStream_t stream);
+
+/* update_GIMP_lp_2d / _3d (gimp.py:428-507).  dmin, dmax: host arrays of 3. */
+int gm_update_lp(int n, int dim, const double* F_new, double* lp, double lp0, const double* x, const double* dmin,
+                 const double* dmax, hipStream_t stream);
+
+/* Block-Jacobi BiCGStab on a node block-CSR matrix (nb = 1..4), FP64
+   (fp32 = 0) or FP32 (fp32 = 1) matrix and vectors, FP64 reductions; x0 = 0;
+   stops when ||r|| <= tol ||b|| (checked every check_every iterations) or
+   after maxiter.  iters, relres: host outputs.  Replaces wps.bsr_set_from_triplets
+   + wp.optim.linear.preconditioner('diag') + bicgstab (qs2d:201-204,
+   qc3d:335-338, 410-412) and the host SciPy / pyamg / pypardiso options
+   (qs2d:205-220, qc3d:339-361). */
+int gm_bicgstab(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
+                const void* b, void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int* iters,
+                double* relres, hipStream_t stream);
+
+/* y = A x on the block-CSR matrix (FP64 accumulation; FP32 or FP64 operands). */
+int gm_bsr_spmv(int nb, int mat_fp32, int vec_fp32, int n_rows, const int* row_ptr, const int* col_idx,
+                const void* vals, const void* x, void* y, hipStream_t stream);
+
+/* out = b - A x in FP64 with A possibly FP32 (iterative-refinement residual);
+   ax: n_rows*nb doubles of scratch. */
+int gm_residual_refine(int nb, int mat_fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
+                       const double* b, const double* x, double* ax, double* out, hipStream_t stream);
+
+/* Deterministic reductions into a device scalar: squared 2-norm and sum
+   (the reference's np.linalg.norm(rhs.numpy()), qs2d:227-231, and the
+   indentation-force atomics, mpm_utils.py:1247-1248).  part >= 2048 doubles. */
+int gm_norm2(long n, const double* v, double* part, double* out, hipStream_t stream);
+int gm_sum(long n, const double* v, double* part, double* out, hipStream_t stream);
+
+/* y += a*x (x FP64 or FP32): from_increment_to_solution (mpm_utils.py:1343-1348). */
+int gm_axpy(long n, double a, const void* x, int x_fp32, double* y, hipStream_t stream);
+
+/* FP64 -> FP32 copy. */
+int gm_d2f(long n, const double* a, float* b, hipStream_t stream);
+
+/* Identity rows for empty diagonals of a scalar (nb = 1) matrix and zero rhs
+   there (nodes that no particle weights; the reference leaves them empty in
+   the consistent-mass system, qc2d:288, qc3d:410). */
+int gm_fix_empty_diag(int n_rows, const int* row_ptr, const int* col_idx, double* vals, double* rhs,
+                      hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEOWARP_AMD_H */

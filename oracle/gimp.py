@@ -74,6 +74,10 @@ def axis_weights(x, lp, h, inv_h):
         wa[two, 1] = 0.5
         wa[two, 2] = a2 / (4.0 * l)
     ext = np.where(one, 2, 3)
+    # a domain ending exactly on a cell face takes the cpGIMP branch with an
+    # exactly zero third node; it adds exact zeros and is left out of the support
+    zero3 = (~one) & (w[:, 2] == 0.0) & (g[:, 2] == 0.0) & (wa[:, 2] == 0.0)
+    ext = np.where(zero3, 2, ext)
     return lo, w, g, wa, ext
 
 

@@ -132,7 +132,10 @@ def return_mapping_DruckerPrager(eps_tr, lam_l, mu, phi, psi, c, shape=0.0, apex
     y = dp_yield(P, Q, phi, c, shape)
     eps = eps_tr.copy()
     plastic = (y >= 1e-10) & (P < 0.0)
-    if apex == 'return':
+    if apex == 'return' and shape > 0.0:
+        # a hyperbolic surface has no apex: every yielding state returns to it
+        plastic = y >= 1e-10
+    elif apex == 'return':
         A, B = _dp_AB(phi, c)
         _, Bg = _dp_AB(psi, c)
         K = lam_l + 2.0 * mu / 3.0

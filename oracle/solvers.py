@@ -156,7 +156,7 @@ class quasi_static_solver_2d(_NewtonMixin):
         self.mat = om.MaterialParams(m['material_name'], m['youngs_modulus'], m['poisson_ratio'],
                                      kappa=m.get('kappa', 0.0), H=m.get('H', 0.0), phi=m.get('friction_angle', 30.0),
                                      psi=m.get('dilation_angle', 0.0), cohesion=m.get('cohesion', 0.0),
-                                     dp_apex=m.get('dp_apex', 'reference'))
+                                     shape=m.get('shape_factor', 0.0), dp_apex=m.get('dp_apex', 'reference'))
         l = load_dict
         self.gravity_mag = l['gravity_mag']
         self.point_load = (l['point_load_value_x'], l['point_load_value_y'])
@@ -241,7 +241,7 @@ class quasi_static_coupled_solver_2d(_NewtonMixin):
         self.mat = om.MaterialParams(m['material_name'], m['youngs_modulus'], m['poisson_ratio'],
                                      kappa=m.get('kappa', 0.0), H=m.get('H', 0.0), phi=m.get('friction_angle', 30.0),
                                      psi=m.get('dilation_angle', 0.0), cohesion=m.get('cohesion', 0.0),
-                                     dp_apex=m.get('dp_apex', 'reference'))
+                                     shape=m.get('shape_factor', 0.0), dp_apex=m.get('dp_apex', 'reference'))
         self.phi_initial = porous_media_dict['phi_initial']
         self.mobility_constant = porous_media_dict['mobility_constant']
         l = load_dict
@@ -347,7 +347,7 @@ class quasi_static_coupled_solver_3d(_NewtonMixin):
         self.mat = om.MaterialParams(m['material_name'], m['youngs_modulus'], m['poisson_ratio'],
                                      kappa=m.get('kappa', 0.0), H=m.get('H', 0.0), phi=m.get('friction_angle', 30.0),
                                      psi=m.get('dilation_angle', 0.0), cohesion=m.get('cohesion', 0.0),
-                                     dp_apex=m.get('dp_apex', 'reference'))
+                                     shape=m.get('shape_factor', 0.0), dp_apex=m.get('dp_apex', 'reference'))
         self.phi_initial = porous_media_dict['phi_initial']
         self.mobility_constant = porous_media_dict['mobility_constant']
         l = load_dict
