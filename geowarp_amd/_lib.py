@@ -72,6 +72,8 @@ _SIGS = {
                              ctypes.POINTER(c_double), _P]),
     'gm_bicgstab': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, ctypes.POINTER(gm_krylov_ws), c_double, c_int,
                             c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_double), _P]),
+    'gm_gmres': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, c_int, c_double, c_int,
+                         ctypes.POINTER(c_int), ctypes.POINTER(c_double), _P]),
     'gm_bsr_spmv': (c_int, [c_int, c_int, c_int, c_int, _P, _P, _P, _P, _P, _P]),
     'gm_norm2': (c_int, [c_long, _P, _P, _P, _P]),
     'gm_sum': (c_int, [c_long, _P, _P, _P, _P]),
@@ -79,6 +81,9 @@ _SIGS = {
     'gm_residual_refine': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
     'gm_d2f': (c_int, [c_long, _P, _P, _P]),
     'gm_fix_empty_diag': (c_int, [c_int, _P, _P, _P, _P, _P]),
+    'gm_gimp_weights': (c_int, [c_int, c_int, _P, _P, ctypes.POINTER(c_double), _P, _P, _P, _P]),
+    'gm_return_map': (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
+                              c_double, c_double, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

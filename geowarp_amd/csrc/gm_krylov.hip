@@ -12,6 +12,10 @@
 //  * Matrix and vectors in FP64 or FP32; reductions always in FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "gm_common.h"
 #include "gm_types.h"
 
@@ -532,4 +536,239 @@ extern "C" int gm_fix_empty_diag(int n_rows, const int* row_ptr, const int* col_
   k_fix_empty_diag<<<gm_blocks(n_rows, 256), 256, 0, s>>>(n_rows, row_ptr, col_idx, vals, rhs);
   GM_LAUNCH_CHECK();
   return 0;
+}
+
+// ================================================================== restarted GMRES(m)
+// Right-preconditioned GMRES(m) (Saad & Schultz 1986) with block-Jacobi M:
+// Arnoldi by classical Gram-Schmidt applied twice (CGS2) on device-resident
+// basis vectors; the small Hessenberg least-squares problem (Givens rotations)
+// is solved on the host, which reads one column of H per iteration.
+// x <- x + M^{-1} V y at the end of every cycle; restart from r = b - A x.
+
+// partial dots of w with basis vectors V[0..k): part[i*nchunk + c]
+template <typename TV>
+__global__ void k_multidot(long n, int k, const TV* V, long ldv, const TV* w, int nchunk, double* part) {
+  __shared__ double sh[RED_T];
+  int i = blockIdx.y;
+  double s = 0.0;
+  const TV* vi = V + (long)i * ldv;
+  for (long r = blockIdx.x * (long)RED_T + threadIdx.x; r < n; r += (long)nchunk * RED_T)
+    s += (double)vi[r] * (double)w[r];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = RED_T / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(long)i * nchunk + blockIdx.x] = sh[0];
+}
+
+// h[i] = sum_c part[i*nchunk + c]  (one block per basis vector, fixed order)
+__global__ void k_multidot_fin(int nchunk, const double* part, double* h) {
+  __shared__ double sh[RED_T];
+  int i = blockIdx.x;
+  double s = 0.0;
+  for (int c = threadIdx.x; c < nchunk; c += RED_T) s += part[(long)i * nchunk + c];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = RED_T / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) h[i] = sh[0];
+}
+
+// w -= sum_i V_i h_i   (h on the device)
+template <typename TV>
+__global__ void k_multiaxpy(long n, int k, const TV* V, long ldv, const double* h, TV* w) {
+  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+    double acc = (double)w[r];
+    for (int i = 0; i < k; ++i) acc -= h[i] * (double)V[(long)i * ldv + r];
+    w[r] = (TV)acc;
+  }
+}
+
+// out = sum_i V_i y_i   (y on the device)
+template <typename TV>
+__global__ void k_combine(long n, int k, const TV* V, long ldv, const double* y, TV* out) {
+  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int i = 0; i < k; ++i) acc += y[i] * (double)V[(long)i * ldv + r];
+    out[r] = (TV)acc;
+  }
+}
+
+template <typename TV>
+__global__ void k_scale_copy(long n, const TV* a, double s, TV* b) {
+  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x)
+    b[r] = (TV)((double)a[r] * s);
+}
+
+template <typename TV>
+__global__ void k_sub_into(long n, const TV* b, const TV* ax, TV* r) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    r[i] = (TV)((double)b[i] - (double)ax[i]);
+}
+
+template <int NB, typename TM, typename TV>
+__global__ void k_apply_dinv_add(int n_rows, const TM* dinv, const TV* in, TV* x) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n_rows) return;
+  double v[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) v[j] = (double)in[row * NB + j];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) s += (double)dinv[(long)row * NB * NB + i * NB + j] * v[j];
+    x[row * NB + i] = (TV)((double)x[row * NB + i] + s);
+  }
+}
+
+template <int NB, typename TM, typename TV>
+__global__ void k_apply_dinv(int n_rows, const TM* dinv, const TV* in, TV* out) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n_rows) return;
+  apply_dinv<NB, TM, TV>(dinv, row, in, out);
+}
+
+template <int NB, typename TM, typename TV>
+static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, const TV* b, TV* x, TV* V, TV* z,
+                      TV* w, TM* dinv, double* part, double* dh, int m, double tol, int maxiter, int* iters_out,
+                      double* relres_out, hipStream_t st) {
+  long n = (long)n_rows * NB;
+  long ldv = n;
+  int rb = gm_blocks(n_rows, 256);
+  int vb = dot_blocks(n);
+  int nchunk = 64;
+  k_diag_inv<NB, TM><<<rb, 256, 0, st>>>(n_rows, rp, ci, vals, dinv);
+  GM_HIP(hipMemsetAsync(x, 0, sizeof(TV) * n, st));
+  // ||b||
+  k_dot2<TV><<<vb, RED_T, 0, st>>>(n, b, b, nullptr, nullptr, part);
+  k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh, nullptr);
+  GM_LAUNCH_CHECK();
+  double hb = 0.0;
+  GM_HIP(hipMemcpyAsync(&hb, dh, sizeof(double), hipMemcpyDeviceToHost, st));
+  GM_HIP(hipStreamSynchronize(st));
+  double bnorm = sqrt(hb);
+  *iters_out = 0;
+  *relres_out = 0.0;
+  if (!(bnorm == bnorm) || bnorm > 1e300) {
+    gm::set_error("gmres: right-hand side is not finite");
+    return -5;
+  }
+  if (bnorm == 0.0) return 0;
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hcol(m + 2);
+  int total = 0;
+  double rel = 1.0;
+  bool first = true;
+  while (total < maxiter) {
+    // r = b - A x  -> V[0]
+    if (first) {
+      GM_HIP(hipMemcpyAsync(V, b, sizeof(TV) * n, hipMemcpyDeviceToDevice, st));
+    } else {
+      launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, x, w, nullptr, nullptr, nullptr, st);
+      k_sub_into<TV><<<vb, RED_T, 0, st>>>(n, b, w, V);
+    }
+    k_dot2<TV><<<vb, RED_T, 0, st>>>(n, V, V, nullptr, nullptr, part);
+    k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh, nullptr);
+    GM_LAUNCH_CHECK();
+    GM_HIP(hipMemcpyAsync(&hb, dh, sizeof(double), hipMemcpyDeviceToHost, st));
+    GM_HIP(hipStreamSynchronize(st));
+    double beta = sqrt(hb);
+    rel = beta / bnorm;
+    if (!(rel == rel)) {
+      gm::set_error("gmres: residual became NaN after %d iterations", total);
+      *iters_out = total;
+      *relres_out = rel;
+      return -5;
+    }
+    if (rel <= tol) break;
+    first = false;
+    k_scale_copy<TV><<<vb, RED_T, 0, st>>>(n, V, 1.0 / beta, V);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int j = 0;
+    for (; j < m && total < maxiter; ++j, ++total) {
+      TV* vj = V + (long)j * ldv;
+      TV* vj1 = V + (long)(j + 1) * ldv;
+      k_apply_dinv<NB, TM, TV><<<rb, 256, 0, st>>>(n_rows, dinv, vj, z);
+      launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, z, w, nullptr, nullptr, nullptr, st);
+      // CGS2
+      for (int pass = 0; pass < 2; ++pass) {
+        k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, st>>>(n, j + 1, V, ldv, w, nchunk, part);
+        k_multidot_fin<<<j + 1, RED_T, 0, st>>>(nchunk, part, dh + pass * (m + 2));
+        k_multiaxpy<TV><<<vb, RED_T, 0, st>>>(n, j + 1, V, ldv, dh + pass * (m + 2), w);
+      }
+      k_dot2<TV><<<vb, RED_T, 0, st>>>(n, w, w, nullptr, nullptr, part);
+      k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + 2 * (m + 2), nullptr);
+      GM_LAUNCH_CHECK();
+      std::vector<double> hh(3 * (m + 2));
+      GM_HIP(hipMemcpyAsync(hh.data(), dh, sizeof(double) * 3 * (m + 2), hipMemcpyDeviceToHost, st));
+      GM_HIP(hipStreamSynchronize(st));
+      for (int i = 0; i <= j; ++i) hcol[i] = hh[i] + hh[(m + 2) + i];
+      double hn = sqrt(fmax(hh[2 * (m + 2)], 0.0));
+      hcol[j + 1] = hn;
+      if (hn > 0.0) k_scale_copy<TV><<<vb, RED_T, 0, st>>>(n, w, 1.0 / hn, vj1);
+      // apply previous rotations, then a new one
+      for (int i = 0; i < j; ++i) {
+        double t = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
+        hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1];
+        hcol[i] = t;
+      }
+      double den = sqrt(hcol[j] * hcol[j] + hcol[j + 1] * hcol[j + 1]);
+      cs[j] = den > 0.0 ? hcol[j] / den : 1.0;
+      sn[j] = den > 0.0 ? hcol[j + 1] / den : 0.0;
+      hcol[j] = den;
+      hcol[j + 1] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hcol[i];
+      rel = fabs(g[j + 1]) / bnorm;
+      if (rel <= tol || hn == 0.0) {
+        ++j;
+        ++total;
+        break;
+      }
+    }
+    // y = H^{-1} g (upper triangular j x j), x += M^{-1} V y
+    for (int i = j - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int k2 = i + 1; k2 < j; ++k2) s -= H[(size_t)i * m + k2] * y[k2];
+      y[i] = s / H[(size_t)i * m + i];
+    }
+    GM_HIP(hipMemcpyAsync(dh, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
+    k_combine<TV><<<vb, RED_T, 0, st>>>(n, j, V, ldv, dh, w);
+    k_apply_dinv_add<NB, TM, TV><<<rb, 256, 0, st>>>(n_rows, dinv, w, x);
+    GM_LAUNCH_CHECK();
+    GM_HIP(hipStreamSynchronize(st));
+    if (rel <= tol) break;
+  }
+  *iters_out = total;
+  *relres_out = rel;
+  return 0;
+}
+
+// V: (m+1)*n_rows*nb entries of the solve precision; z, w: n_rows*nb each;
+// dinv: n_rows*nb*nb; part >= 64*(m+1) doubles and >= 2048; dh >= 3*(m+2) doubles.
+extern "C" int gm_gmres(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
+                        const void* b, void* x, void* V, void* z, void* w, void* dinv, double* part, double* dh, int m,
+                        double tol, int maxiter, int* iters, double* relres, hipStream_t s) {
+  GM_REQUIRE(iters && relres && m >= 1 && m <= 256, "gm_gmres: bad argument");
+#define GM_GMRES(NB)                                                                                             \
+  if (nb == NB)                                                                                                   \
+    return fp32 ? gmres_impl<NB, float, float>(n_rows, row_ptr, col_idx, (const float*)vals, (const float*)b,      \
+                                               (float*)x, (float*)V, (float*)z, (float*)w, (float*)dinv, part, dh, \
+                                               m, tol, maxiter, iters, relres, s)                                  \
+                : gmres_impl<NB, double, double>(n_rows, row_ptr, col_idx, (const double*)vals, (const double*)b,  \
+                                                 (double*)x, (double*)V, (double*)z, (double*)w, (double*)dinv,    \
+                                                 part, dh, m, tol, maxiter, iters, relres, s);
+  GM_GMRES(1)
+  GM_GMRES(2)
+  GM_GMRES(3)
+  GM_GMRES(4)
+#undef GM_GMRES
+  gm::set_error("gm_gmres: unsupported block size %d", nb);
+  return -1;
 }

@@ -1125,3 +1125,90 @@ extern "C" int gm_update_lp(int n, int dim, const double* F_new, double* lp, dou
   GM_LAUNCH_CHECK();
   return 0;
 }
+
+// ================================================================== standalone entry points
+// Per-particle cpGIMP data in the reference's output layout
+// (get_GIMP_shape_function_and_gradient_2d/3d, gimp.py:25-112, 197-315, and the
+// _avg_ variants :115-193, :318-425): out[p] = (w0_x, w0_y[, w0_z], w1_..., w2_...,
+// g0_..., g1_..., g2_...) and avg[p] likewise (weights then gradients), base[p*dim+a] = lo.
+__global__ void k_gimp_weights(int n, int dim, const double* x, const double* lp, double h0, double h1, double h2,
+                               double* out, double* avg, int* base) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double hs[3] = {h0, h1, h2};
+  for (int a = 0; a < dim; ++a) {
+    Axis ax = gimp_axis(x[p * dim + a], lp[p * dim + a], hs[a], 1.0 / hs[a]);
+    for (int k = 0; k < 3; ++k) {
+      out[(long)p * 6 * dim + k * dim + a] = ax.w[k];
+      out[(long)p * 6 * dim + 3 * dim + k * dim + a] = ax.g[k];
+      if (avg) {
+        avg[(long)p * 6 * dim + k * dim + a] = ax.wa[k];
+        // the reference's average gradients: -sign(x - x_k)/(4 lp) at k = 0, 2 in the cpGIMP branch,
+        // with sign(0) = +1 as Warp's wp.sign defines it
+        double gk = 0.0;
+        double xa = x[p * dim + a], la = lp[p * dim + a], ih = 1.0 / hs[a];
+        bool cp = (int)((xa - la) * ih + 1e-8) != (int)((xa + la) * ih + 1e-8);  // cpGIMP branch
+        if (cp && k != 1) {
+          double d = xa - (ax.lo + k) * hs[a];
+          gk = -1.0 / (4.0 * la) * (d < 0.0 ? -1.0 : 1.0);
+        }
+        avg[(long)p * 6 * dim + 3 * dim + k * dim + a] = gk;
+      }
+    }
+    if (base) base[p * dim + a] = ax.lo;
+  }
+}
+
+extern "C" int gm_gimp_weights(int n, int dim, const double* x, const double* lp, const double* h, double* out,
+                               double* avg, int* base, hipStream_t s) {
+  GM_REQUIRE(h && x && lp && out, "gm_gimp_weights: null argument");
+  if (n == 0) return 0;
+  k_gimp_weights<<<gm_blocks(n, 256), 256, 0, s>>>(n, dim, x, lp, h[0], h[1], dim == 3 ? h[2] : h[0], out, avg,
+                                                   base);
+  GM_LAUNCH_CHECK();
+  return 0;
+}
+
+// Standalone principal return mapping (plasticity.py:8-37 J2; triaxial_DruckerPrager.py:308-412 DP):
+// eps_tr, eps (n x 3 principal log strains), dE (n x 9 consistent tangent d eps/d eps_tr, may be null),
+// alpha_old/new (J2 hardening, may be null).
+__global__ void k_return_map(int n, MatParams m, const double* etr, double* eps, double* dE, const double* ao,
+                             double* an) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double e[3], d[9], a_new = 0.0;
+  double et[3] = {etr[3 * p], etr[3 * p + 1], etr[3 * p + 2]};
+  if (m.kind == MAT_J2) {
+    return_j2(et, m, ao ? ao[p] : 0.0, e, d, a_new);
+  } else if (m.kind == MAT_DP) {
+    return_dp(et, m, e, d);
+  } else {
+    for (int i = 0; i < 3; ++i) e[i] = et[i];
+    for (int i = 0; i < 9; ++i) d[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  }
+  for (int i = 0; i < 3; ++i) eps[3 * p + i] = e[i];
+  if (dE)
+    for (int i = 0; i < 9; ++i) dE[9 * p + i] = d[i];
+  if (an) an[p] = a_new;
+}
+
+extern "C" int gm_return_map(int n, int mat_kind, int dp_apex, double lam, double mu, double kappa, double H,
+                             double phi, double psi, double coh, double shape, const double* eps_tr, double* eps,
+                             double* dE, const double* alpha_old, double* alpha_new, hipStream_t s) {
+  if (n == 0) return 0;
+  MatParams m;
+  m.kind = mat_kind;
+  m.dp_apex = dp_apex;
+  m.lam = lam;
+  m.mu = mu;
+  m.E = 0.0;
+  m.kappa = kappa;
+  m.H = H;
+  m.phi = phi;
+  m.psi = psi;
+  m.coh = coh;
+  m.shape = shape;
+  k_return_map<<<gm_blocks(n, 256), 256, 0, s>>>(n, m, eps_tr, eps, dE, alpha_old, alpha_new);
+  GM_LAUNCH_CHECK();
+  return 0;
+}

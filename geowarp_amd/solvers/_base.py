@@ -11,7 +11,9 @@ Extensions over the reference (all optional, defaults reproduce it):
   iteration_dict['tol_mode']      'absolute' (reference, qs2d:230) or 'relative'
   iteration_dict['precision']     'fp64' (default) or 'mixed' (FP32 Jacobian +
                                   FP32 Krylov, FP64 residual/refinement)
-  iteration_dict['linear_tol'], ['linear_maxiter'], ['ir_steps']
+  iteration_dict['linear_tol'], ['linear_maxiter'], ['ir_steps'], ['forcing']
+                                  ('fixed' | 'ew': Eisenstat-Walker inner tolerance),
+  ['krylov'] ('gmres' default | 'bicgstab'), ['krylov_restart']
   iteration_dict['verbose']       print residual norms like the reference
   material_dict['H']              J2 linear isotropic hardening modulus
   material_dict['friction_angle'], ['dilation_angle'], ['cohesion'],
@@ -87,9 +89,13 @@ class _QuasiStaticBase:
         self.solver_name = it.get('solver_name', 'warp')
         self.tol_mode = it.get('tol_mode', 'absolute')
         self.precision = it.get('precision', 'fp64')
-        self.linear_tol = it.get('linear_tol', 1e-10 if self.precision == 'fp64' else 1e-4)
+        self.linear_tol = it.get('linear_tol', 1e-10 if self.precision == 'fp64' else 1e-6)
         self.linear_maxiter = it.get('linear_maxiter', 20000)
         self.ir_steps = it.get('ir_steps', 1)
+        self.krylov_method = it.get('krylov', 'gmres')
+        self.krylov_restart = it.get('krylov_restart', 40)
+        # inner tolerance: 'fixed' (linear_tol) or 'ew' (Eisenstat-Walker choice 2)
+        self.forcing = it.get('forcing', 'fixed' if self.precision == 'fp64' else 'ew')
         self.verbose = it.get('verbose', False)
         m = material_dict
         self.material_name = m['material_name']
@@ -197,6 +203,8 @@ class _QuasiStaticBase:
         r0 = None
         it = 0
         tm = eng.timers
+        eta = None
+        nr_prev = None
         for it in range(self.n_iter):
             if tm is not None:
                 tm.mark('start')
@@ -212,11 +220,17 @@ class _QuasiStaticBase:
             eng.jacobian(x, lp)
             if tm is not None:
                 tm.mark('node_jacobian')
-            eng.solve(self.linear_tol, self.linear_maxiter, ir_steps=self.ir_steps)
+            nr = math.sqrt(max(eng.scal[0].item(), 0.0))
+            tol_lin = self.linear_tol
+            if self.forcing == 'ew':
+                eta = self._ew_forcing(nr, nr_prev, eta, r0 if r0 is not None else nr)
+                tol_lin = eta
+            eng.solve(tol_lin, self.linear_maxiter, ir_steps=self.ir_steps, method=self.krylov_method,
+                      restart=self.krylov_restart)
             eng.axpy(1.0, eng.delta, u_new)
             if tm is not None:
                 tm.mark('krylov')
-            nr = math.sqrt(max(eng.scal[0].item(), 0.0))
+            nr_prev = nr
             self.residual_history.append(nr)
             if self.verbose:
                 print('residual.norm:', nr)
@@ -228,6 +242,23 @@ class _QuasiStaticBase:
             if crit < self.tol:
                 break
         self.newton_iterations = it + 1
+
+    def _ew_forcing(self, nr, nr_prev, eta_prev, r0):
+        """Eisenstat & Walker (1996) choice 2 forcing term with their safeguards,
+        clamped to [linear_tol, 0.1], and not tighter than needed to reach the
+        Newton tolerance."""
+        gamma, alpha = 0.9, 2.0
+        if nr_prev is None or eta_prev is None:
+            eta = 1e-2
+        else:
+            eta = gamma * (nr / nr_prev) ** alpha
+            sg = gamma * eta_prev ** alpha
+            if sg > 0.1:
+                eta = max(eta, sg)
+        target = self.tol * r0 if self.tol_mode == 'relative' else self.tol
+        if nr > 0:
+            eta = max(eta, 0.5 * target / nr)
+        return min(max(eta, self.linear_tol), 0.1)
 
     # ------------------------------------------------------------------ dof-major views
     def _sync_flags_out(self):

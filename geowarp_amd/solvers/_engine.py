@@ -152,7 +152,7 @@ class Engine:
         return p
 
     # ------------------------------------------------------------------ step phases
-    def begin_step(self, x, lp):
+    def begin_step(self, x, lp, pattern=True):
         """Bin particles by stencil base node and build the block-CSR pattern."""
         lib = L.lib()
         self.ensure_particles(x.shape[0])
@@ -163,6 +163,8 @@ class Engine:
         if nbad:
             raise L.GMError('%d particle stencils leave the background grid (or are not finite)' % nbad)
         self.nnzb = 0
+        if not pattern:
+            return
         for _ in range(3):
             pat = self._pattern()
             rc = lib.gm_bsr_pattern(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ctypes.byref(pat),
@@ -249,24 +251,50 @@ class Engine:
                                          L.ptr(self.vals), int(fp32), int(self.coupled), _stream()),
                 'gm_node_jacobian')
 
-    def solve(self, tol, maxiter, check_every=8, ir_steps=1):
-        """Solve K delta = rhs (K = -dR/du).  FP64: BiCGStab in FP64.  Mixed:
-        BiCGStab in FP32 on the FP32 Jacobian, wrapped in `ir_steps` sweeps of
+    def gmres_workspace(self, nb, fp32, m):
+        key = ('gmres', nb, fp32, m)
+        if key not in self._ws:
+            dt = torch.float32 if fp32 else torch.float64
+            n = self.n_nodes * nb
+            V = torch.zeros((m + 1) * n, dtype=dt, device=self.dev)
+            z = torch.zeros(n, dtype=dt, device=self.dev)
+            w = torch.zeros(n, dtype=dt, device=self.dev)
+            dinv = torch.zeros(self.n_nodes * nb * nb, dtype=dt, device=self.dev)
+            part = torch.zeros(max(64 * (m + 1), 2048), dtype=torch.float64, device=self.dev)
+            dh = torch.zeros(3 * (m + 2), dtype=torch.float64, device=self.dev)
+            self._ws[key] = (V, z, w, dinv, part, dh)
+        return self._ws[key]
+
+    def krylov(self, b, x, fp32, tol, maxiter, method='gmres', restart=40, check_every=8):
+        """One Krylov solve K x = b on the current Jacobian (block-Jacobi
+        preconditioned GMRES(restart) or BiCGStab); returns (iterations, relres)."""
+        lib = L.lib()
+        it = L.c_int(0)
+        rr = L.c_double(0.0)
+        if method == 'gmres':
+            V, z, w, dinv, part, dh = self.gmres_workspace(self.nb, fp32, restart)
+            L.check(lib.gm_gmres(self.nb, int(fp32), self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
+                                 L.ptr(self.vals), L.ptr(b), L.ptr(x), L.ptr(V), L.ptr(z), L.ptr(w), L.ptr(dinv),
+                                 L.ptr(part), L.ptr(dh), restart, tol, maxiter, L.ctypes.byref(it),
+                                 L.ctypes.byref(rr), _stream()), 'gm_gmres')
+        else:
+            ws = self.workspace(self.nb, fp32)
+            L.check(lib.gm_bicgstab(self.nb, int(fp32), self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
+                                    L.ptr(self.vals), L.ptr(b), L.ptr(x), L.ctypes.byref(ws), tol, maxiter,
+                                    check_every, L.ctypes.byref(it), L.ctypes.byref(rr), _stream()), 'gm_bicgstab')
+        return it.value, rr.value
+
+    def solve(self, tol, maxiter, check_every=8, ir_steps=1, method='gmres', restart=40):
+        """Solve K delta = rhs (K = -dR/du).  FP64: Krylov in FP64.  Mixed:
+        Krylov in FP32 on the FP32 Jacobian, wrapped in `ir_steps` sweeps of
         iterative refinement whose residual rhs - K delta is formed in FP64."""
         lib = L.lib()
         n = self.n_nodes * self.nb
-        it = L.c_int(0)
-        rr = L.c_double(0.0)
         if self.precision == 'fp64':
-            ws = self.workspace(self.nb, False)
-            L.check(lib.gm_bicgstab(self.nb, 0, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
-                                    L.ptr(self.vals), L.ptr(self.rhs), L.ptr(self.delta), L.ctypes.byref(ws), tol,
-                                    maxiter, check_every, L.ctypes.byref(it), L.ctypes.byref(rr), _stream()),
-                    'gm_bicgstab')
-            self.stats['krylov_iters'].append(it.value)
-            self.stats['krylov_relres'].append(rr.value)
+            its, rel = self.krylov(self.rhs, self.delta, False, tol, maxiter, method, restart, check_every)
+            self.stats['krylov_iters'].append(its)
+            self.stats['krylov_relres'].append(rel)
             return
-        ws = self.workspace(self.nb, True)
         if not hasattr(self, '_b32'):
             self._b32 = torch.zeros(n, dtype=torch.float32, device=self.dev)
             self._x32 = torch.zeros(n, dtype=torch.float32, device=self.dev)
@@ -283,14 +311,11 @@ class Engine:
                                                L.ptr(self._res64), _stream()), 'gm_residual_refine')
                 src = self._res64
             L.check(lib.gm_d2f(n, L.ptr(src), L.ptr(self._b32), _stream()), 'gm_d2f')
-            L.check(lib.gm_bicgstab(self.nb, 1, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
-                                    L.ptr(self.vals), L.ptr(self._b32), L.ptr(self._x32), L.ctypes.byref(ws), tol,
-                                    maxiter, check_every, L.ctypes.byref(it), L.ctypes.byref(rr), _stream()),
-                    'gm_bicgstab(fp32)')
-            total += it.value
+            its, rel = self.krylov(self._b32, self._x32, True, tol, maxiter, method, restart, check_every)
+            total += its
             L.check(lib.gm_axpy(n, 1.0, L.ptr(self._x32), 1, L.ptr(self.delta), _stream()), 'gm_axpy')
         self.stats['krylov_iters'].append(total)
-        self.stats['krylov_relres'].append(rr.value)
+        self.stats['krylov_relres'].append(rel)
 
     def axpy(self, a, x, y):
         L.check(L.lib().gm_axpy(x.numel(), float(a), L.ptr(x), int(x.dtype == torch.float32), L.ptr(y), _stream()),

@@ -123,6 +123,15 @@ int gm_bicgstab(int nb, int fp32, int n_rows, const int* row_ptr, const int* col
                 const void* b, void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int* iters,
                 double* relres, hipStream_t stream);
 
+/* Restarted right-preconditioned GMRES(m) with block-Jacobi and CGS2 Arnoldi on
+   the same matrices (the robust alternative to BiCGStab, no breakdown).  V:
+   (m+1)*n_rows*nb entries of the solve precision, z and w: n_rows*nb each,
+   dinv: n_rows*nb*nb, part: >= max(64*(m+1), 2048) doubles, dh: >= 3*(m+2)
+   doubles.  Replaces the same reference solves as gm_bicgstab. */
+int gm_gmres(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals, const void* b,
+             void* x, void* V, void* z, void* w, void* dinv, double* part, double* dh, int m, double tol, int maxiter,
+             int* iters, double* relres, hipStream_t stream);
+
 /* y = A x on the block-CSR matrix (FP64 accumulation; FP32 or FP64 operands). */
 int gm_bsr_spmv(int nb, int mat_fp32, int vec_fp32, int n_rows, const int* row_ptr, const int* col_idx,
                 const void* vals, const void* x, void* y, hipStream_t stream);
@@ -149,6 +158,25 @@ int gm_d2f(long n, const double* a, float* b, hipStream_t stream);
    the consistent-mass system, qc2d:288, qc3d:410). */
 int gm_fix_empty_diag(int n_rows, const int* row_ptr, const int* col_idx, double* vals, double* rhs,
                       hipStream_t stream);
+
+/* Per-particle cpGIMP weights and gradients in the reference's output order
+   (get_GIMP_shape_function_and_gradient_2d/3d, gimp.py:25-112, 197-315):
+   out[p] = (w0 per axis, w1 per axis, w2 per axis, g0 per axis, g1, g2), 6*dim
+   doubles; avg (may be null) likewise for the cell-average variants
+   (gimp.py:115-193, 318-425); base (may be null) = first stencil node per axis.
+   h: host array of dim spacings (2D: dx twice, as gimp.py:26-31). */
+int gm_gimp_weights(int n, int dim, const double* x, const double* lp, const double* h, double* out, double* avg,
+                    int* base, hipStream_t stream);
+
+/* Standalone principal-strain return mapping, one particle per lane:
+   mat_kind 1 = J2 (plasticity.py:8-37, with linear hardening H),
+   3 = Drucker-Prager (triaxial_DruckerPrager.py:308-412; dp_apex 0 = the
+   reference's no-return for P_trial >= 0, 1 = apex return).  eps_tr, eps: n x 3;
+   dE (may be null): n x 9 consistent tangent d eps / d eps_tr; alpha_old/new
+   (may be null): J2 equivalent plastic strain. */
+int gm_return_map(int n, int mat_kind, int dp_apex, double lam, double mu, double kappa, double H, double phi,
+                  double psi, double coh, double shape, const double* eps_tr, double* eps, double* dE,
+                  const double* alpha_old, double* alpha_new, hipStream_t stream);
 
 #ifdef __cplusplus
 }
