@@ -45,6 +45,11 @@ class gm_stage_args(ctypes.Structure):
                 ('footing', c_double * 4), ('footing_height', c_double), ('pen_par', c_double), ('penalty', c_int)]
 
 
+class gm_norsand_params(ctypes.Structure):
+    _fields_ = [(k, c_double) for k in ('kappa', 'tilde_lambda', 'M', 'N', 'beta', 'vc0', 'p0', 'v0', 'h', 'mu',
+                                        'ev0', 'tol')]
+
+
 class gm_krylov_ws(ctypes.Structure):
     _fields_ = [('r', c_void_p), ('rhat', c_void_p), ('p', c_void_p), ('v', c_void_p), ('s', c_void_p),
                 ('t', c_void_p), ('y', c_void_p), ('z', c_void_p), ('dinv', c_void_p), ('part', c_void_p),
@@ -84,6 +89,7 @@ _SIGS = {
     'gm_gimp_weights': (c_int, [c_int, c_int, _P, _P, ctypes.POINTER(c_double), _P, _P, _P, _P]),
     'gm_return_map': (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
                               c_double, c_double, _P, _P, _P, _P, _P, _P]),
+    'gm_return_map_norsand': (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -1212,3 +1212,123 @@ extern "C" int gm_return_map(int n, int mat_kind, int dp_apex, double lam, doubl
   GM_LAUNCH_CHECK();
   return 0;
 }
+
+// ---------------------------------------------------------------------------------------------
+// NorSand return mapping (triaxial_NorSand.py:155-400), one particle per lane.  Pressure-dependent
+// hyperelasticity P = p0 exp(-(eps_v - ev0)/kappa), Q = 3 mu eps_s; yield Q + eta(P, pi) P; fixed
+// 15-iteration local Newton on (eps_v, eps_s, dlambda, pi_i) with hardening h (pi_i - pi*).
+// Tension (eps_v > tol) returns zero strain; no return when the yield value is below tol.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+__device__ inline double ns_eta(double M, double N, double P, double pi) {
+  if (fabs(N) < 1e-10) return M * (1.0 + log(pi / P));
+  return (M / N) * (1.0 - (1.0 - N) * pow(P / pi, N / (1.0 - N)));
+}
+__device__ inline double ns_deta_dP(double M, double N, double P, double pi) {
+  if (fabs(N) < 1e-10) return -M / P;
+  return -M * pow(P / pi, (2.0 * N - 1.0) / (1.0 - N)) / pi;
+}
+__device__ inline double ns_deta_dpi(double M, double N, double P, double pi) {
+  if (fabs(N) < 1e-10) return M / pi;
+  return M * pow(P / pi, (2.0 * N - 1.0) / (1.0 - N)) * P / (pi * pi);
+}
+__device__ inline double ns_eps_s(const double* e) {
+  double a = e[0] - e[1], b = e[1] - e[2], c = e[0] - e[2];
+  return sqrt(2.0 / 9.0 * (a * a + b * b + c * c));
+}
+
+__global__ void k_return_map_norsand(int n, gm_norsand_params q, const double* etr, const double* ev_total,
+                                     const double* pi_old, double* eps, double* pi_new, double* res_last) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double et[3] = {etr[3 * p], etr[3 * p + 1], etr[3 * p + 2]};
+  const double po = pi_old[p];
+  const double alpha_bar = -3.5 / q.beta;
+  const double ev = et[0] + et[1] + et[2];
+  const double es = ns_eps_s(et);
+  double nd[3] = {0.0, 0.0, 0.0};
+  if (es > q.tol) {
+    const double s23 = sqrt(2.0 / 3.0);
+    for (int i = 0; i < 3; ++i) nd[i] = s23 * (et[i] - ev / 3.0) / es;
+  }
+  double Pt = q.p0 * exp(-(ev - q.ev0) / q.kappa);
+  double Qt = 3.0 * q.mu * es;
+  const double new_v = exp(ev_total[p]) * q.v0;
+  double e[3] = {et[0], et[1], et[2]};
+  double pi = po;
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  if (ev > q.tol) {
+    e[0] = e[1] = e[2] = 0.0;
+  } else if (Qt + ns_eta(q.M, q.N, Pt, po) * Pt >= q.tol) {
+    double dl = 0.0;
+    const double M = q.M, N = q.N;
+    for (int it = 0; it < 15; ++it) {
+      const double erv = e[0] + e[1] + e[2];
+      const double ers = ns_eps_s(e);
+      const double dEdP = ns_deta_dP(M, N, Pt, pi);
+      const double dEdPi = ns_deta_dpi(M, N, Pt, pi);
+      const double eta = ns_eta(M, N, Pt, pi);
+      const double dFdP = (eta - M) / (1.0 - N);
+      const double psi_i = new_v - q.vc0 + q.tilde_lambda * log(-pi);
+      // pi* / P: the yield-test tolerance picks the branch for pi*, 1e-10 for its derivative
+      // (triaxial_NorSand.py:356-362 and the derivative helpers)
+      const double pstar = fabs(N) < q.tol ? Pt * exp(alpha_bar * psi_i / M)
+                                           : Pt * pow(1.0 - alpha_bar * psi_i * N / M, (N - 1.0) / N);
+      const double dPs_dP = fabs(N) < 1e-10 ? exp(alpha_bar * psi_i / M)
+                                            : pow(1.0 - alpha_bar * psi_i * N / M, (N - 1.0) / N);
+      r[0] = erv - ev + dl * q.beta * dFdP;
+      r[1] = ers - es + dl;
+      r[2] = Qt + eta * Pt;
+      r[3] = pi - po + q.h * (pi - pstar) * dl;
+      const double dPdev = q.p0 * exp(-(erv - q.ev0) / q.kappa) / (-q.kappa);
+      const double d2F_dPdev = dEdP * dPdev / (1.0 - N);
+      const double d2F_dPdpi = dEdPi / (1.0 - N);
+      const double dFdev = (dEdP * Pt + eta) * dPdev;
+      const double dFdes = 3.0 * q.mu;
+      const double dFdpi = Pt * dEdPi;
+      const double dPsi_dpi = q.tilde_lambda / pi;
+      double dPs_dpi;
+      if (fabs(N) < 1e-10)
+        dPs_dpi = Pt * exp(alpha_bar * psi_i / M) * alpha_bar / M * dPsi_dpi;
+      else
+        dPs_dpi = Pt * (N - 1.0) / N * pow(1.0 / (1.0 - alpha_bar * psi_i * N / M), 1.0 / N) *
+                  (-alpha_bar * N / M) * dPsi_dpi;
+      const double dHdev = -q.h * dl * dPs_dP * dPdev;
+      const double dHdpi = 1.0 + q.h * dl * (1.0 - dPs_dpi);
+      double J[4][4] = {{1.0 + dl * q.beta * d2F_dPdev, 0.0, q.beta * dFdP, dl * q.beta * d2F_dPdpi},
+                        {0.0, 1.0, 1.0, 0.0},
+                        {dFdev, dFdes, 0.0, dFdpi},
+                        {dHdev, 0.0, q.h * (pi - pstar), dHdpi}};
+      double x[4] = {r[0], r[1], r[2], r[3]};
+      solve4(J, x);
+      const double nev = erv - x[0];
+      const double nes = ers - x[1];
+      dl -= x[2];
+      pi -= x[3];
+      const double s32 = sqrt(1.5);
+      for (int i = 0; i < 3; ++i) e[i] = nev / 3.0 + nes * nd[i] * s32;
+      const double erv2 = e[0] + e[1] + e[2];
+      Pt = q.p0 * exp(-(erv2 - q.ev0) / q.kappa);
+      Qt = 3.0 * q.mu * ns_eps_s(e);
+    }
+  }
+  for (int i = 0; i < 3; ++i) eps[3 * p + i] = e[i];
+  pi_new[p] = pi;
+  if (res_last)
+    for (int i = 0; i < 4; ++i) res_last[4 * p + i] = r[i];
+}
+
+}  // namespace
+
+extern "C" int gm_return_map_norsand(int n, const gm_norsand_params* prm, const double* eps_tr,
+                                     const double* eps_total_trace, const double* pi_old, double* eps, double* pi_new,
+                                     double* res_last, hipStream_t s) {
+  GM_REQUIRE(prm, "gm_return_map_norsand: null parameters");
+  if (n == 0) return 0;
+  GM_REQUIRE(eps_tr && eps_total_trace && pi_old && eps && pi_new, "gm_return_map_norsand: null argument");
+  k_return_map_norsand<<<gm_blocks(n, 128), 128, 0, s>>>(n, *prm, eps_tr, eps_total_trace, pi_old, eps, pi_new,
+                                                         res_last);
+  GM_LAUNCH_CHECK();
+  return 0;
+}

@@ -85,6 +85,15 @@ typedef struct gm_krylov_ws {
   double* sc;     /* >= 16 */
 } gm_krylov_ws;
 
+/* NorSand parameters (triaxial_NorSand.py:243-262): kappa, tilde_lambda, M, N,
+   beta, critical specific volume vc0, reference pressure p0 (negative in
+   compression), initial specific volume v0, hardening modulus h, shear modulus
+   mu, initial volumetric strain ev0, and the tolerance of the tension / yield
+   tests. */
+typedef struct gm_norsand_params {
+  double kappa, tilde_lambda, M, N, beta, vc0, p0, v0, h, mu, ev0, tol;
+} gm_norsand_params;
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,5 +1,5 @@
 """Return mappings with the reference's names (GeoWarp materials/plasticity.py and
-the Drucker-Prager function of problems/material_tests/triaxial_DruckerPrager.py),
+the Drucker-Prager and NorSand functions of problems/material_tests/triaxial_*.py),
 one particle per lane on the device.
 
 Inputs follow the reference: trial strains as (N, 3, 3) principal (diagonal)
@@ -51,3 +51,27 @@ def return_mapping_DruckerPrager(trial_strain, lame_lambda, lame_mu, friction_an
     out, dE, _ = _call(3, trial_strain, lame_lambda, lame_mu, phi=friction_angle, psi=dilation_angle, c=cohesion,
                        shape=shape_factor, apex={'reference': 0, 'return': 1}[apex], tangent=tangent)
     return (out, dE) if tangent else out
+
+
+def return_mapping_NorSand(kappa, tilde_lambda, M, N, beta, vc0, p0, v0, h, lame_mu, initial_volumetric_strain,
+                           old_pi_array, trial_strain, total_strain, tol=1e-10, return_residual=False):
+    """triaxial_NorSand.py:243-400 for tensors of particles: trial_strain and
+    total_strain as (N, 3, 3) principal matrices (or (N, 3)), old_pi_array (N,).
+    Returns (real_strain, new_pi_array) in the input's form, plus the local
+    residual of the last Newton iterate (N, 4) with return_residual=True (the
+    reference keeps it as saved_local_residual)."""
+    et = _principal(trial_strain).to(torch.float64)
+    tt = _principal(total_strain).to(torch.float64)
+    n = et.shape[0]
+    ev_total = tt.sum(1).contiguous()
+    pi_old = old_pi_array.to(torch.float64).contiguous()
+    eps = torch.empty_like(et)
+    pi_new = torch.empty_like(pi_old)
+    res = torch.empty(n, 4, dtype=torch.float64, device=et.device) if return_residual else None
+    prm = L.gm_norsand_params(kappa, tilde_lambda, M, N, beta, vc0, p0, v0, h, lame_mu, initial_volumetric_strain,
+                              tol)
+    L.check(L.lib().gm_return_map_norsand(n, L.ctypes.byref(prm), L.ptr(et), L.ptr(ev_total), L.ptr(pi_old),
+                                          L.ptr(eps), L.ptr(pi_new), L.ptr(res), _stream()),
+            'gm_return_map_norsand')
+    out = torch.diag_embed(eps) if trial_strain.dim() == 3 else eps
+    return (out, pi_new, res) if return_residual else (out, pi_new)

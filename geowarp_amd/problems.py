@@ -15,8 +15,8 @@ Configurations (the sizes in brackets are the full benchmark ones):
               over 20 steps, roller walls, fixed base.
   bar_j2      2D J2 bar, 0.5 m x 2 m on 256 x 1024 cells of 1.953125 mm, 2x2 PPC
               (1,048,576 particles), E 10 MPa, nu 0.3, sigma_y 20 kPa, H 100 kPa,
-              top-platen displacement 10% of the height in 50 steps (imposed as a
-              penalty platen on the top particles).
+              top-platen displacement 10% of the height in 50 steps (prescribed
+              y increments on the node rows at and above the current top).
   terzaghi    2D u-p column 1 m x 10 m on 32 x 320 cells, 2x2 PPC (40,960
               particles), E 10 MPa, nu 0.3, k/mu_f 1e-4 m^2/(kPa s), phi0 0.5,
               10 kPa surface traction, drained top, undrained base.
@@ -80,9 +80,28 @@ def bar_nh(n_cells_x=8, n_cells_y=80, dx=0.0125, total_steps=20, tol=1e-10):
     )
 
 
-def bar_j2(n_cells_x=256, n_cells_y=1024, dx=1.953125e-3, total_steps=50, tol=1e-8, precision='fp64'):
-    """Platen displacement is imposed by pushing the top particle layer with the
-    gravity-free point-load path replaced by a kinematic platen in the driver."""
+def make_platen(total_disp):
+    """Displacement-controlled rigid platen on top of a 2D bar: every load step,
+    the node rows at and above the current top of the particle domains
+    (max y + lp_y) are fixed in y with the increment total_disp / total_steps;
+    x stays free (frictionless platen).  Driver hook for quasi_static_solver_2d's
+    load_dict['prescribed_increment']."""
+
+    def hook(x, lp, boundary, pres, n_grid_x, n_nodes, dx, current_step, total_steps):
+        nx1 = n_grid_x + 1
+        top = float((x[:, 1] + lp[:, 1]).max())
+        row = int(np.ceil(top / dx - 1e-8))
+        by = _view2d(boundary, 1, n_nodes, nx1)
+        py = _view2d(pres, 1, n_nodes, nx1)
+        by[row:, :] = True
+        py[row:, :] = total_disp / float(total_steps)
+
+    return hook
+
+
+def bar_j2(n_cells_x=256, n_cells_y=1024, dx=1.953125e-3, total_steps=50, tol=1e-8, precision='fp64',
+           platen_strain=0.1):
+    """J2 bar compressed by a top platen by platen_strain of its height."""
     g = _grid2d(n_cells_x, n_cells_y, dx)
     PPD = 2
     npart = n_cells_x * n_cells_y * PPD * PPD
@@ -94,7 +113,8 @@ def bar_j2(n_cells_x=256, n_cells_y=1024, dx=1.953125e-3, total_steps=50, tol=1e
         material_dict=dict(youngs_modulus=1.0e4, poisson_ratio=0.3, material_name='J2',
                            kappa=np.sqrt(2.0 / 3.0) * 20.0, H=100.0),
         load_dict=dict(gravity_mag=10.0, traction_value_x=0.0, traction_value_y=0.0, point_load_value_x=0.0,
-                       point_load_value_y=0.0),
+                       point_load_value_y=0.0,
+                       prescribed_increment=make_platen(-platen_strain * n_cells_y * dx) if platen_strain else None),
         set_boundary_dofs=make_bar_bc(1, 1 + n_cells_x, 1),
         total_steps=total_steps,
     )

@@ -23,6 +23,10 @@ class quasi_static_solver_2d(_QuasiStaticBase):
         self.traction_value_y = l.get('traction_value_y', 0.0)
         self.point_load_value_x = l.get('point_load_value_x', 0.0)
         self.point_load_value_y = l.get('point_load_value_y', 0.0)
+        # optional driver hook for non-zero Dirichlet increments (e.g. a displacement-controlled platen):
+        # f(x_particles, GIMP_lp, boundary_flag_array, prescribed, n_grid_x, n_nodes, dx, current_step,
+        #   total_steps) sets boundary flags and writes prescribed increments into the dof-major `prescribed`
+        self.prescribed_increment = l.get('prescribed_increment')
         self._inc = torch.zeros(self.n_matrix_size, dtype=torch.float64, device=self.device)  # node-interleaved
 
     @property
@@ -49,9 +53,16 @@ class quasi_static_solver_2d(_QuasiStaticBase):
         """reset + P2G of advance_one_step (qs2d:240-245)."""
         self.reset()
         eng = self.engine
+        pres = None
+        if self.prescribed_increment is not None:
+            pres = torch.zeros(self.n_matrix_size, dtype=torch.float64, device=self.device)
+            self.prescribed_increment(self.x_particles, self.GIMP_lp, self.dofStruct.boundary_flag_array, pres,
+                                      self.n_grid_x, self.n_nodes, self.dx, current_step, total_steps)
         eng.begin_step(self.x_particles, self.GIMP_lp)
         eng.set_boundary_from_dofmajor(self.dofStruct.boundary_flag_array)
         eng.p2g(self.x_particles, self.GIMP_lp, -1.0)  # P2G_2d: every support node, no threshold
+        if pres is not None:
+            self._inc.copy_(eng.from_dofmajor(pres * self.dofStruct.boundary_flag_array))
         self._sync_flags_out()
 
     def advance_one_step(self, current_step, total_steps):

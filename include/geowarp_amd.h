@@ -178,6 +178,15 @@ int gm_return_map(int n, int mat_kind, int dp_apex, double lam, double mu, doubl
                   double psi, double coh, double shape, const double* eps_tr, double* eps, double* dE,
                   const double* alpha_old, double* alpha_new, hipStream_t stream);
 
+/* NorSand principal-strain return mapping, one particle per lane
+   (return_mapping_NorSand, triaxial_NorSand.py:243-400): eps_tr n x 3 trial
+   principal strains, eps_total_trace n total volumetric strains (specific
+   volume update), pi_old n image pressures; writes eps (n x 3), pi_new (n) and,
+   if res_last is not null, the 4 local residuals of the last Newton iterate
+   (n x 4, the reference's saved_local_residual[14]). */
+int gm_return_map_norsand(int n, const gm_norsand_params* prm, const double* eps_tr, const double* eps_total_trace,
+                          const double* pi_old, double* eps, double* pi_new, double* res_last, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -160,6 +160,7 @@ class quasi_static_solver_2d(_NewtonMixin):
         l = load_dict
         self.gravity_mag = l['gravity_mag']
         self.point_load = (l['point_load_value_x'], l['point_load_value_y'])
+        self.prescribed_increment = l.get('prescribed_increment')  # driver hook, see prepare_step
         if x_particles is None:
             x_particles = om.init_particles_rectangle_2d(self.start_x, self.start_y, self.end_x, self.end_y,
                                                          self.dx, self.n_grid_x, self.n_grid_y, self.PPD)
@@ -197,6 +198,12 @@ class quasi_static_solver_2d(_NewtonMixin):
     def prepare_step(self, current_step, total_steps):
         """reset + P2G (qs2d:240-245); returns (stencil, residual fn, u0, FD steps)."""
         self.reset()
+        if self.prescribed_increment is not None:
+            # non-zero Dirichlet increments set by the driver (fixed dofs keep them: their rhs is 0)
+            pres = np.zeros(self.n_matrix_size)
+            self.prescribed_increment(self.x_particles, self.lp, self.boundary, pres, self.n_grid_x, self.n_nodes,
+                                      self.dx, current_step, total_steps)
+            self.increment_solution = pres * self.boundary
         st = og.Stencil(self.x_particles, self.lp, 2, (self.dx,), self.n_grid)
         self.act = om.p2g_2d_activate(st, self.n_nodes)
         steps = (1e-6 * self.dx, 1e-6 * self.dx)

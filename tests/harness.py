@@ -12,6 +12,8 @@ MINI = {
                                          dict(material_name='Hencky elasticity')),
     'bar_j2': lambda: _j2_mini(),
     'bar_dp': lambda: _dp_mini(),
+    'bar_j2_platen': lambda: P.bar_j2(n_cells_x=4, n_cells_y=16, dx=0.0125, total_steps=10, tol=1e-9,
+                                      platen_strain=0.02),
     'terzaghi': lambda: P.terzaghi(n_cells_x=2, n_cells_y=16, width=0.125, total_steps=10, dt=1.0),
     'indent_nh': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.05, footing=0.5,
                                     material='Neo-Hookean', tol=1e-10),

@@ -252,3 +252,96 @@ def test_p2g_residual_g2p_match_oracle():
     xref, _ = omp.g2p(st, x, du, n_nodes, 2)
     assert H.rel_err(xt.cpu().numpy(), xref) < 1e-14
     assert H.rel_err(Fo.cpu().numpy(), Fn) < 1e-13
+
+
+@pytest.mark.gpu
+def test_return_mapping_norsand_matches_oracle():
+    """Device NorSand against the oracle on trial states that cover tension,
+    elastic and plastic cases; the converged plastic states also satisfy the
+    reference's local equations (triaxial_NorSand.py:338-343) to round-off."""
+    from geowarp_amd.materials import plasticity as pl
+    prm = dict(kappa=0.0015, tilde_lambda=0.02, M=1.27, N=0.4, beta=1.0, vc0=1.8911, p0=-390.0, v0=1.75, h=70.0,
+               mu=4.5e4, ev0=0.0)
+    pc = -715.0
+    pi0 = pc * (1 - 0.4) ** ((1 - 0.4) / 0.4)
+    rng = np.random.default_rng(8)
+    n = 64
+    e = np.stack([rng.uniform(-2e-4, -1e-5, n), rng.uniform(-2e-4, -1e-5, n), rng.uniform(-8e-3, -1e-5, n)], 1)
+    e[:4] = np.array([[1e-3, 1e-3, 1e-3], [-1e-6, -1e-6, -1e-6], [-1e-4, -1e-4, -8e-3], [-1e-4, -1.5e-4, -6e-3]])
+    tot = e.sum(1) * 1.01
+    pi_old = np.full(n, pi0)
+    ref_e, ref_pi, ref_res = om.return_mapping_NorSand(e, tot, pi_old, prm)
+    dev = _dev()
+    tens = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    out, pin, res = pl.return_mapping_NorSand(prm['kappa'], prm['tilde_lambda'], prm['M'], prm['N'], prm['beta'],
+                                              prm['vc0'], prm['p0'], prm['v0'], prm['h'], prm['mu'], prm['ev0'],
+                                              tens(pi_old), tens(e), torch.diag_embed(tens(tot[:, None] / 3 *
+                                                                                           np.ones((1, 3)))),
+                                              return_residual=True)
+    out = out.cpu().numpy()
+    pin = pin.cpu().numpy()
+    np.testing.assert_allclose(out, ref_e, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(pin, ref_pi, rtol=1e-9)
+    assert np.all(out[0] == 0.0)                   # tension: zero strain
+    conv = np.abs(ref_res[:, :3]).max(1) < 1e-8     # cases where the fixed 15 iterations converged
+    assert conv.sum() >= 2
+    assert np.abs(res.cpu().numpy()[conv, :3]).max() < 1e-8
+
+
+@pytest.mark.gpu
+def test_e_grad_matches_fd_of_loss_at_fixed_solution():
+    """E_grad of the 3D solver (d loss / d E with the nodal solution held, as
+    the reference's tape gives it at qc3d:323-325) against a central FD of the
+    loss, where the force comes from the module entry point
+    assemble_residual_coupled_3d_neohookean at the converged solution and E +- h."""
+    import geowarp_amd.solvers as gs
+    from geowarp_amd.mpm import mpm_utils as mu_
+    from geowarp_amd.solvers.quasi_static_coupled_solver_3d import get_indentation_loss
+    from geowarp_amd import problems as P
+    cfg = P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=2, depth=0.01, footing=0.5, material='Neo-Hookean',
+                     tol=1e-10, n_iter=12)
+    s = H.make(gs, cfg)
+    captured = {}
+    eng = s.engine
+    g2p = eng.g2p
+
+    def capture(*a, **k):
+        captured.update(x=s.x_particles.clone(), lp=s.GIMP_lp.clone(), F=s.deformation_gradient_total_old.clone(),
+                        old=s.old_solution.clone(), new=s.new_solution.clone(),
+                        bnd=s.dofStruct.boundary_flag_array.clone(), act=s.dofStruct.activate_flag_array.clone())
+        return g2p(*a, **k)
+
+    eng.g2p = capture
+    for k in range(2):
+        s.advance_one_step(k, 2)
+    assert s.E_grad is not None
+    g = cfg['background_grid_dict']
+    l = cfg['load_dict']
+    dev = s.x_particles.device
+    n = s.n_particles
+
+    def force(E):
+        F = captured['F'].clone()
+        Fn, bo, bn = F.clone(), F.clone(), F.clone()
+        sig = torch.zeros(n, 3, 3, dtype=torch.float64, device=dev)
+        rhs = torch.zeros(4 * s.n_nodes, dtype=torch.float64, device=dev)
+        f = torch.zeros(1, dtype=torch.float64, device=dev)
+        fa = torch.zeros(2, dtype=torch.float64, device=dev)
+        mu_.assemble_residual_coupled_3d_neohookean(
+            captured['x'], g['dx'], g['dy'], g['dz'], 1 / g['dx'], 1 / g['dy'], 1 / g['dz'], s.dt, g['n_grid_x'],
+            g['n_grid_y'], s.n_nodes, s.PPD, captured['old'], captured['new'], F, Fn, bo, bn, None, None, sig,
+            s.phi_initial, s.mobility_constant, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, l['footing_region_min_x'],
+            l['footing_region_max_x'], l['footing_region_min_y'], l['footing_region_max_y'],
+            l['footing_initial_height'], l['footing_incr_disp'], l['penalty_factor'], 1, 2,
+            torch.tensor([E], dtype=torch.float64), 0.3, s.p_vol, s.p_rho, captured['lp'], captured['bnd'],
+            captured['act'], rhs, f, fa)
+        return float(f.item())
+
+    E = s.youngs_modulus
+    f0 = force(E)
+    assert abs(f0 - float(s.indentation_force.item())) <= 1e-6 * abs(f0)
+    h = 1e-3 * E
+    lp_ = get_indentation_loss(force(E + h), l['footing_incr_disp'], 2.0)
+    lm_ = get_indentation_loss(force(E - h), l['footing_incr_disp'], 2.0)
+    fd = (lp_ - lm_) / (2 * h)
+    assert abs(float(s.E_grad[0]) - fd) <= 1e-6 * abs(fd)
