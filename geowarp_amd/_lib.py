@@ -42,12 +42,23 @@ class gm_stage_args(ctypes.Structure):
                 ('p_vol', c_double), ('p_rho', c_double), ('phi0', c_double), ('mobility', c_double),
                 ('dt', c_double), ('PPD', c_double),
                 ('gravity', c_double * 3), ('point_load', c_double * 3), ('traction', c_double * 3),
-                ('footing', c_double * 4), ('footing_height', c_double), ('pen_par', c_double), ('penalty', c_int)]
+                ('footing', c_double * 4), ('footing_height', c_double), ('pen_par', c_double), ('penalty', c_int),
+                ('p_phi', c_void_p), ('p_coh', c_void_p)]
 
 
 class gm_norsand_params(ctypes.Structure):
     _fields_ = [(k, c_double) for k in ('kappa', 'tilde_lambda', 'M', 'N', 'beta', 'vc0', 'p0', 'v0', 'h', 'mu',
                                         'ev0', 'tol')]
+
+
+class gm_local_args(ctypes.Structure):
+    _fields_ = [('n_particles', c_int), ('nx1', c_int), ('mat_kind', c_int), ('dp_apex', c_int),
+                ('h', c_double), ('shape', c_double), ('p_vol', c_double), ('p_rho', c_double),
+                ('gravity', c_double * 2), ('point_load', c_double * 2), ('lp0', c_double),
+                ('dmin', c_double * 2), ('dmax', c_double * 2)] + \
+        [(k, c_double) for k in ('E', 'nu', 'kappa', 'H', 'phi', 'psi', 'coh')] + \
+        [(k, c_void_p) for k in ('p_phi', 'p_coh', 'x', 'lp', 'F_old', 'b_old', 'alpha_old', 'ext_flag', 'u',
+                                 'cot', 'cot_scale', 'xbar', 'lpbar', 'Fbar', 'bbar', 'abar')]
 
 
 class gm_krylov_ws(ctypes.Structure):
@@ -90,6 +101,9 @@ _SIGS = {
     'gm_return_map': (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
                               c_double, c_double, _P, _P, _P, _P, _P, _P]),
     'gm_return_map_norsand': (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'gm_local_eval_2d': (c_int, [_P, _P, _P, _P]),
+    'gm_local_vjp_2d': (c_int, [_P, _P, _P]),
+    'gm_bsr_transpose': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -431,9 +431,9 @@ __global__ void __launch_bounds__(128) k_particle_stage(gm_grid g, gm_stage_args
   mp.E = a.E;
   mp.kappa = a.kappa;
   mp.H = a.H;
-  mp.phi = a.phi;
+  mp.phi = a.p_phi ? a.p_phi[p] : a.phi;
   mp.psi = a.psi;
-  mp.coh = a.coh;
+  mp.coh = a.p_coh ? a.p_coh[p] : a.coh;
   mp.shape = a.shape;
   M3 tau, bn;
   SpectralState ss;

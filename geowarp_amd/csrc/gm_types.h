@@ -73,6 +73,8 @@ typedef struct gm_stage_args {
   double footing_height; /* current indenter height */
   double pen_par;        /* penalty stiffness E*pf*(dx/PPD)(dy/PPD) */
   int penalty;           /* 3D indenter active */
+  const double* p_phi;   /* per-particle Drucker-Prager friction angle, may be null (then phi) */
+  const double* p_coh;   /* per-particle cohesion, may be null (then coh) */
 } gm_stage_args;
 
 /* Krylov workspace (device pointers).  Vectors hold n_rows*nb entries of the
@@ -93,6 +95,38 @@ typedef struct gm_krylov_ws {
 typedef struct gm_norsand_params {
   double kappa, tilde_lambda, M, N, beta, vc0, p0, v0, h, mu, ev0, tol;
 } gm_norsand_params;
+
+/* Arguments of the 2D plane-strain local map used by the adjoint
+   (gm_local_eval_2d / gm_local_vjp_2d).  Particle arrays are in particle order;
+   nodal vectors (u, cot) are node-interleaved (value c of node n at 2n+c).
+   Optional pointers may be null (zero cotangent / uniform parameter). */
+typedef struct gm_local_args {
+  int n_particles;
+  int nx1;                 /* n_grid_x + 1 */
+  int mat_kind, dp_apex;   /* 0 Hencky, 1 J2, 2 Neo-Hookean, 3 Drucker-Prager */
+  double h;                /* dx (used on both axes) */
+  double shape;
+  double p_vol, p_rho;
+  double gravity[2], point_load[2];  /* ramped by (step+1)/total */
+  double lp0, dmin[2], dmax[2];
+  double E, nu, kappa, H, phi, psi, coh;
+  const double* p_phi;     /* per-particle friction angle (optional) */
+  const double* p_coh;     /* per-particle cohesion (optional) */
+  const double* x;         /* n x 2 */
+  const double* lp;        /* n x 2 */
+  const double* F_old;     /* n x 9 */
+  const double* b_old;     /* n x 9 */
+  const double* alpha_old; /* n (optional) */
+  const uint8_t* ext_flag; /* n (optional) */
+  const double* u;         /* converged nodal increment, 2 per node */
+  const double* cot;       /* nodal cotangent of the residual contributions (optional) */
+  const double* cot_scale; /* n per-particle factor on that cotangent (optional) */
+  const double* xbar;      /* n x 2 cotangent of x_new (optional) */
+  const double* lpbar;     /* n x 2 (optional) */
+  const double* Fbar;      /* n x 5: xx, xy, yx, yy, zz (optional) */
+  const double* bbar;      /* n x 4: xx, xy (symmetric pair), yy, zz (optional) */
+  const double* abar;      /* n (optional) */
+} gm_local_args;
 
 #ifdef __cplusplus
 }

@@ -243,6 +243,8 @@ def indent3d(n_cells=(64, 64, 32), dx=1.0 / 64, total_steps=40, depth=0.05, foot
 def build_solver(module, cfg, **kw):
     """Instantiate cfg['solver'] from `module` (geowarp_amd.solvers or oracle.solvers)."""
     cls = getattr(module, cfg['solver'])
+    if 'x_particles' in cfg:
+        kw.setdefault('x_particles', cfg['x_particles'])
     if cfg['solver'] == 'quasi_static_solver_2d':
         s = cls(cfg['background_grid_dict'], cfg['particles_dict'], cfg['iteration_dict'], cfg['material_dict'],
                 cfg['load_dict'], **kw)
@@ -250,3 +252,113 @@ def build_solver(module, cfg, **kw):
         s = cls(cfg['background_grid_dict'], cfg['particles_dict'], cfg['iteration_dict'], cfg['material_dict'],
                 cfg['porous_media_dict'], cfg['load_dict'], cfg['set_boundary_dofs'], **kw)
     return s
+
+
+def _like(ref, arr):
+    """numpy bool/float array as the array type of `ref` (numpy or torch)."""
+    if isinstance(ref, np.ndarray):
+        return arr
+    import torch
+    return torch.as_tensor(arr, device=ref.device)
+
+
+def _to_np(a):
+    return a if isinstance(a, np.ndarray) else a.detach().cpu().numpy()
+
+
+def specimen_counts(n_specimens=64, seed=0, min_count=1e3, max_count=5e4, phi_range=(25.0, 40.0),
+                    coh_range=(0.0, 20.0)):
+    """Specimen sizes and true parameters of the batched inverse analysis:
+    numpy default_rng(seed), drawn in the order counts, phi, c; count
+    N = exp(U(ln min, ln max)) gives W = round(sqrt(N / 8)) cells wide, 2W tall
+    (8 W^2 particles at 2x2 per cell)."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(min_count), np.log(max_count), n_specimens))
+    phi = rng.uniform(phi_range[0], phi_range[1], n_specimens)
+    coh = rng.uniform(coh_range[0], coh_range[1], n_specimens)
+    W = np.maximum(1, np.round(np.sqrt(counts / 8.0)).astype(int))
+    return W, phi, coh
+
+
+def specimens2d(widths, phi, coh, dx=0.01, total_steps=30, strain=0.03, E=1.0e4, nu=0.1, gap=3, tol=1e-10,
+                n_iter=12, precision='fp64'):
+    """Independent 2D plane-strain Drucker-Prager specimens (W_i x 2W_i cells,
+    2x2 particles per cell) side by side on one background grid, `gap` empty
+    cells apart.  Each specimen sits in a box with frictionless walls (x fixed
+    on its side node lines), a fixed base and a top platen that moves down by
+    strain * height over total_steps load steps (prescribed y increments on the
+    node rows at and above its current top).  Per-particle friction angle and
+    cohesion come from the specimen (solver.friction_angle_array /
+    cohesion_array); `groups` maps particles to specimens."""
+    widths = [int(w) for w in widths]
+    PPD = 2
+    pad_lo, pad_hi = 1, 3
+    x0 = []
+    cx = pad_lo + gap
+    for w in widths:
+        x0.append(cx)
+        cx += w + gap
+    nx = cx + pad_lo
+    hmax = 2 * max(widths)
+    ny = hmax + pad_lo + pad_hi
+    g = dict(n_grid_x=nx, n_grid_y=ny, max_x=nx * dx, max_y=ny * dx, dx=dx, dy=dx, start_x=0.0, end_x=nx * dx,
+             start_y=pad_lo * dx, end_y=(pad_lo + hmax) * dx, domain_min_x=0.0, domain_max_x=nx * dx,
+             domain_min_y=pad_lo * dx, domain_max_y=ny * dx)
+    xs, grp = [], []
+    for i, w in enumerate(widths):
+        sx, sy = x0[i] * dx, pad_lo * dx
+        ii, jj, a, b = np.meshgrid(np.arange(w), np.arange(2 * w), np.arange(PPD), np.arange(PPD), indexing='ij')
+        px = sx + (ii + (0.5 + a) / PPD) * dx
+        py = sy + (jj + (0.5 + b) / PPD) * dx
+        xs.append(np.stack([px.reshape(-1), py.reshape(-1)], 1))
+        grp.append(np.full(px.size, i))
+    x = np.concatenate(xs)
+    groups = np.concatenate(grp)
+    nx1 = nx + 1
+    # column ownership: node columns x0 .. x0+w belong to specimen i
+    owner = np.full(nx1, -1)
+    for i, w in enumerate(widths):
+        owner[x0[i]:x0[i] + w + 1] = i
+    heights = np.array([2 * w * dx for w in widths])
+    dy_step = -strain * heights / float(total_steps)
+
+    def set_boundary_dofs(boundary, n_grid_x, n_nodes):
+        bx = _view2d(boundary, 0, n_nodes, nx1)
+        by = _view2d(boundary, 1, n_nodes, nx1)
+        inner = np.zeros(nx1, dtype=bool)
+        for i, w in enumerate(widths):
+            inner[x0[i] + 1:x0[i] + w] = True
+        bx[:, _like(bx, ~inner)] = True
+        by[:pad_lo + 1, :] = True
+
+    def platen(xp, lp, boundary, pres, n_grid_x, n_nodes, dx_, current_step, total_steps_):
+        top = _to_np(xp[:, 1] + lp[:, 1])
+        tops = np.full(len(widths), -np.inf)
+        np.maximum.at(tops, groups, top)
+        rows = np.ceil(tops / dx_ - 1e-8).astype(int)
+        iy = np.arange(ny + 1)[:, None]
+        col_row = np.where(owner >= 0, rows[np.maximum(owner, 0)], ny + 2)
+        m = (iy >= col_row[None, :]) & (owner[None, :] >= 0)
+        val = np.where(owner >= 0, dy_step[np.maximum(owner, 0)] * total_steps / float(total_steps_), 0.0)
+        by = _view2d(boundary, 1, n_nodes, nx1)
+        py = _view2d(pres, 1, n_nodes, nx1)
+        by[_like(by, m)] = True
+        py[_like(py, m)] = _like(py, np.broadcast_to(val[None, :], m.shape)[m])
+
+    n = x.shape[0]
+    return dict(
+        solver='quasi_static_solver_2d',
+        background_grid_dict=g,
+        particles_dict=dict(n_particles=n, PPD=PPD, p_vol=(dx / PPD) ** 2, p_rho=2.0),
+        iteration_dict=dict(n_iter=n_iter, tol=tol, tol_mode='relative', precision=precision),
+        material_dict=dict(youngs_modulus=E, poisson_ratio=nu, material_name='Drucker-Prager',
+                           friction_angle=float(np.mean(phi)), dilation_angle=0.0, cohesion=float(np.mean(coh))),
+        load_dict=dict(gravity_mag=0.0, traction_value_x=0.0, traction_value_y=0.0, point_load_value_x=0.0,
+                       point_load_value_y=0.0, prescribed_increment=platen),
+        set_boundary_dofs=set_boundary_dofs,
+        total_steps=total_steps,
+        x_particles=x,
+        groups=groups,
+        phi=np.asarray(phi, dtype=np.float64),
+        coh=np.asarray(coh, dtype=np.float64),
+    )

@@ -184,6 +184,11 @@ class _QuasiStaticBase:
         a.lam, a.mu, a.E = self.lame_lambda, self.lame_mu, self.youngs_modulus
         a.kappa, a.H = self.kappa, self.hardening
         a.phi, a.psi, a.coh, a.shape = self.friction_angle, self.dilation_angle, self.cohesion, self.shape_factor
+        # optional per-particle Drucker-Prager parameters (batched specimens)
+        pa = getattr(self, 'friction_angle_array', None)
+        ca = getattr(self, 'cohesion_array', None)
+        a.p_phi = pa.data_ptr() if pa is not None else None
+        a.p_coh = ca.data_ptr() if ca is not None else None
         a.p_vol, a.p_rho = self.p_vol, self.p_rho
         a.PPD = float(self.PPD)
         s1 = (float(current_step) + 1.0) / float(total_steps)

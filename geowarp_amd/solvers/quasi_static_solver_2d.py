@@ -61,6 +61,7 @@ class quasi_static_solver_2d(_QuasiStaticBase):
         eng.begin_step(self.x_particles, self.GIMP_lp)
         eng.set_boundary_from_dofmajor(self.dofStruct.boundary_flag_array)
         eng.p2g(self.x_particles, self.GIMP_lp, -1.0)  # P2G_2d: every support node, no threshold
+        self._prescribed = pres
         if pres is not None:
             self._inc.copy_(eng.from_dofmajor(pres * self.dofStruct.boundary_flag_array))
         self._sync_flags_out()
@@ -68,8 +69,12 @@ class quasi_static_solver_2d(_QuasiStaticBase):
     def advance_one_step(self, current_step, total_steps):
         """qs2d:239-258: reset, P2G activation, Newton, G2P, lp update."""
         self.prepare_step(current_step, total_steps)
-        eng = self.engine
         self.newton_iter(current_step, total_steps)
+        self.finish_step()
+
+    def finish_step(self):
+        """G2P and the lp update that close a load step (qs2d:247-258)."""
+        eng = self.engine
         eng.g2p(self.x_particles, self.GIMP_lp, self._inc, None, self.deformation_gradient_total_old,
                 self.deformation_gradient_total_new, self.left_Cauchy_Green_old, self.left_Cauchy_Green_new,
                 self.plastic_strain_old, self.plastic_strain_new, None)

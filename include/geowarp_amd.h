@@ -187,6 +187,32 @@ int gm_return_map(int n, int mat_kind, int dp_apex, double lam, double mu, doubl
 int gm_return_map_norsand(int n, const gm_norsand_params* prm, const double* eps_tr, const double* eps_total_trace,
                           const double* pi_old, double* eps, double* pi_new, double* res_last, hipStream_t stream);
 
+/* ---------------------------------------------------------------- adjoint (2D plane strain)
+   No single reference function corresponds: the reference differentiates its
+   load step with the Warp tape (qs2d:158-192 reverse sweeps; qc3d:270-327 for
+   the E gradient).  These entry points are the pieces of a hand-written
+   adjoint of quasi_static_solver_2d's converged load step. */
+
+/* Per-particle local map (residual contributions assemble_residual_2d_*,
+   mpm_utils.py:416-780; G2P_2d :1351-1419; update_GIMP_lp_2d gimp.py:428-462)
+   evaluated in double: acc[p] = sum_I cot_I . r_I^p (may be null); state (may be
+   null): 18 doubles per particle = x_new(2), lp_new(2), F_new(xx,xy,yx,yy,zz),
+   b_new(xx,xy,yy,zz), alpha_new, sigma(xx,xy,yy,zz). */
+int gm_local_eval_2d(const gm_local_args* args, double* acc, double* state, hipStream_t stream);
+
+/* Vector-Jacobian products of the local map: g[p*25 + d] = d/d(input d) of
+   sum_I cot_I . r_I^p + xbar.x_new + lpbar.lp_new + Fbar:F_new + bbar:b_new +
+   abar alpha_new, for the 25 inputs 0-1 x, 2-3 lp, 4-8 F_old(xx,xy,yx,yy,zz),
+   9-12 b_old(xx,xy=yx,yy,zz), 13 alpha_old, 14-17 H(xx,xy,yx,yy), 18-24
+   E, nu, kappa, H, phi, psi, cohesion.  Forward-mode dual numbers, one
+   direction per input. */
+int gm_local_vjp_2d(const gm_local_args* args, double* g, hipStream_t stream);
+
+/* K^T in K's block pattern (structurally symmetric, ascending columns per row),
+   FP64 (fp32 = 0) or FP32 values; used for the transposed (adjoint) solve. */
+int gm_bsr_transpose(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
+                     void* vals_t, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

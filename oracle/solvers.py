@@ -178,6 +178,7 @@ class quasi_static_solver_2d(_NewtonMixin):
         self.boundary = np.zeros(self.n_matrix_size, dtype=bool)
         self.act = np.zeros(self.n_matrix_size, dtype=bool)
         self.increment_solution = np.zeros(self.n_matrix_size)
+        self.reaction_history = []
 
     def reset(self):
         """qs2d:144-147 (boundary flags are kept)."""
@@ -204,6 +205,7 @@ class quasi_static_solver_2d(_NewtonMixin):
             self.prescribed_increment(self.x_particles, self.lp, self.boundary, pres, self.n_grid_x, self.n_nodes,
                                       self.dx, current_step, total_steps)
             self.increment_solution = pres * self.boundary
+            self._prescribed = pres
         st = og.Stencil(self.x_particles, self.lp, 2, (self.dx,), self.n_grid)
         self.act = om.p2g_2d_activate(st, self.n_nodes)
         steps = (1e-6 * self.dx, 1e-6 * self.dx)
@@ -214,6 +216,15 @@ class quasi_static_solver_2d(_NewtonMixin):
         st, f, u0, steps = self.prepare_step(current_step, total_steps)
         u = self._newton(f, None, u0, steps)
         self.increment_solution = u
+        if self.prescribed_increment is not None:
+            # reaction of the displacement-controlled dofs: the unmasked residual summed over them
+            r, _, _, _, _ = om.residual_2d(st, self.n_nodes, u, self.F_old, self.b_old, self.alpha_old,
+                                           self.external_flag, self.mat, self.p_vol, self.p_rho, self.gravity_mag,
+                                           self.point_load, current_step, total_steps,
+                                           np.zeros(self.n_matrix_size, dtype=bool),
+                                           np.ones(self.n_matrix_size, dtype=bool))
+            m = (self._prescribed != 0) & self.boundary
+            self.reaction_history.append(float(r[m].sum()))
         self.x_particles, _ = om.g2p(st, self.x_particles, u, self.n_nodes, 2)
         self.F_old = self.F_new.copy()
         self.b_old = self.b_new.copy()
