@@ -118,10 +118,12 @@ def _dp_hess(S, Q, psi, c, shape):
     return dFdQ[:, None, None] * hessQ + d2[:, None, None] * gQ[:, :, None] * gQ[:, None, :]
 
 
-def return_mapping_DruckerPrager(eps_tr, lam_l, mu, phi, psi, c, shape=0.0, apex='reference'):
+def return_mapping_DruckerPrager(eps_tr, lam_l, mu, phi, psi, c, shape=0.0, apex='reference', n_iter=10):
     """return_mapping_DruckerPrager (triaxial_DruckerPrager.py:308-412).
 
-    Fixed 10-iteration local Newton on (eps1, eps2, eps3, dlambda).  No return
+    Fixed 10-iteration local Newton on (eps1, eps2, eps3, dlambda) (n_iter
+    other than 10 exists for tests that separate converged returns from
+    unconverged ones).  No return
     when f < 1e-10 or when P_trial >= 0 (the reference skips the apex,
     :345-349).  ``apex='return'`` replaces that skip by a return to the apex
     P = A/B (textbook; not in the reference).
@@ -153,7 +155,7 @@ def return_mapping_DruckerPrager(eps_tr, lam_l, mu, phi, psi, c, shape=0.0, apex
         e = e_tr.copy()
         dl = np.zeros(idx.size)
         t, Pp, Sp, Qp = _dp_PQ(e, lam_l, mu)
-        for _ in range(10):
+        for _ in range(n_iter):
             gg = _dp_grad(t, Qp, psi, c, shape)
             gf = _dp_grad(t, Qp, phi, c, shape)
             gfe = gf @ a_el  # grad_f[a]*a[a,b] (:229-238)
