@@ -633,16 +633,33 @@ __global__ void k_apply_dinv(int n_rows, const TM* dinv, const TV* in, TV* out) 
   apply_dinv<NB, TM, TV>(dinv, row, in, out);
 }
 
+// Block-Jacobi right preconditioner: out = Dinv in; x += Dinv in.
 template <int NB, typename TM, typename TV>
+struct JacobiPrec {
+  int n_rows;
+  const TM* dinv;
+  int apply(const TV* in, TV* out, hipStream_t st) const {
+    k_apply_dinv<NB, TM, TV><<<gm_blocks(n_rows, 256), 256, 0, st>>>(n_rows, dinv, in, out);
+    GM_LAUNCH_CHECK();
+    return 0;
+  }
+  int apply_add(const TV* in, TV* x, TV* /*tmp*/, hipStream_t st) const {
+    k_apply_dinv_add<NB, TM, TV><<<gm_blocks(n_rows, 256), 256, 0, st>>>(n_rows, dinv, in, x);
+    GM_LAUNCH_CHECK();
+    return 0;
+  }
+};
+
+// Restarted GMRES(m), right preconditioned (prec.apply / prec.apply_add), CGS2
+// Arnoldi with device dot products, Givens rotations on the host.
+template <int NB, typename TM, typename TV, typename Prec>
 static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, const TV* b, TV* x, TV* V, TV* z,
-                      TV* w, TM* dinv, double* part, double* dh, int m, double tol, int maxiter, int* iters_out,
-                      double* relres_out, hipStream_t st) {
+                      TV* w, const Prec& prec, double* part, double* dh, int m, double tol, int maxiter,
+                      int* iters_out, double* relres_out, hipStream_t st) {
   long n = (long)n_rows * NB;
   long ldv = n;
-  int rb = gm_blocks(n_rows, 256);
   int vb = dot_blocks(n);
   int nchunk = 64;
-  k_diag_inv<NB, TM><<<rb, 256, 0, st>>>(n_rows, rp, ci, vals, dinv);
   GM_HIP(hipMemsetAsync(x, 0, sizeof(TV) * n, st));
   // ||b||
   k_dot2<TV><<<vb, RED_T, 0, st>>>(n, b, b, nullptr, nullptr, part);
@@ -693,7 +710,8 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
     for (; j < m && total < maxiter; ++j, ++total) {
       TV* vj = V + (long)j * ldv;
       TV* vj1 = V + (long)(j + 1) * ldv;
-      k_apply_dinv<NB, TM, TV><<<rb, 256, 0, st>>>(n_rows, dinv, vj, z);
+      int prc = prec.apply(vj, z, st);
+      if (prc) return prc;
       launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, z, w, nullptr, nullptr, nullptr, st);
       // CGS2
       for (int pass = 0; pass < 2; ++pass) {
@@ -740,8 +758,9 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
     }
     GM_HIP(hipMemcpyAsync(dh, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
     k_combine<TV><<<vb, RED_T, 0, st>>>(n, j, V, ldv, dh, w);
-    k_apply_dinv_add<NB, TM, TV><<<rb, 256, 0, st>>>(n_rows, dinv, w, x);
     GM_LAUNCH_CHECK();
+    int prc = prec.apply_add(w, x, z, st);
+    if (prc) return prc;
     GM_HIP(hipStreamSynchronize(st));
     if (rel <= tol) break;
   }
@@ -756,19 +775,493 @@ extern "C" int gm_gmres(int nb, int fp32, int n_rows, const int* row_ptr, const 
                         const void* b, void* x, void* V, void* z, void* w, void* dinv, double* part, double* dh, int m,
                         double tol, int maxiter, int* iters, double* relres, hipStream_t s) {
   GM_REQUIRE(iters && relres && m >= 1 && m <= 256, "gm_gmres: bad argument");
-#define GM_GMRES(NB)                                                                                             \
-  if (nb == NB)                                                                                                   \
-    return fp32 ? gmres_impl<NB, float, float>(n_rows, row_ptr, col_idx, (const float*)vals, (const float*)b,      \
-                                               (float*)x, (float*)V, (float*)z, (float*)w, (float*)dinv, part, dh, \
-                                               m, tol, maxiter, iters, relres, s)                                  \
-                : gmres_impl<NB, double, double>(n_rows, row_ptr, col_idx, (const double*)vals, (const double*)b,  \
-                                                 (double*)x, (double*)V, (double*)z, (double*)w, (double*)dinv,    \
-                                                 part, dh, m, tol, maxiter, iters, relres, s);
+#define GM_GMRES(NB)                                                                                          \
+  if (nb == NB) {                                                                                              \
+    if (fp32) {                                                                                                \
+      k_diag_inv<NB, float><<<gm_blocks(n_rows, 256), 256, 0, s>>>(n_rows, row_ptr, col_idx, (const float*)vals, \
+                                                                   (float*)dinv);                              \
+      JacobiPrec<NB, float, float> pc{n_rows, (const float*)dinv};                                             \
+      return gmres_impl<NB, float, float>(n_rows, row_ptr, col_idx, (const float*)vals, (const float*)b,        \
+                                          (float*)x, (float*)V, (float*)z, (float*)w, pc, part, dh, m, tol,     \
+                                          maxiter, iters, relres, s);                                          \
+    }                                                                                                          \
+    k_diag_inv<NB, double><<<gm_blocks(n_rows, 256), 256, 0, s>>>(n_rows, row_ptr, col_idx, (const double*)vals, \
+                                                                  (double*)dinv);                              \
+    JacobiPrec<NB, double, double> pc{n_rows, (const double*)dinv};                                            \
+    return gmres_impl<NB, double, double>(n_rows, row_ptr, col_idx, (const double*)vals, (const double*)b,      \
+                                          (double*)x, (double*)V, (double*)z, (double*)w, pc, part, dh, m, tol, \
+                                          maxiter, iters, relres, s);                                          \
+  }
   GM_GMRES(1)
   GM_GMRES(2)
   GM_GMRES(3)
   GM_GMRES(4)
 #undef GM_GMRES
   gm::set_error("gm_gmres: unsupported block size %d", nb);
+  return -1;
+}
+
+// ================================================================== geometric multigrid
+// Preconditioner for the Newton systems: V-cycles on a hierarchy of background
+// grids (every second node per axis), trilinear (bilinear in 2D) prolongation P
+// restricted to free dofs, Galerkin coarse operators A_c = P^T A P built on the
+// device in two gathers (AP per fine row, then P^T(AP) per coarse row, fixed
+// summation order), damped block-Jacobi smoothing with the exact nb x nb
+// diagonal-block inverses.  The cycle is a fixed linear operator, so it serves as
+// the right preconditioner of GMRES.
+namespace mg {
+
+__device__ __forceinline__ int floordiv2(int a) { return (a >= 0) ? (a >> 1) : -((-a + 1) >> 1); }
+
+template <int DIM>
+__device__ __forceinline__ void coords(int node, int nx, int ny, int* c) {
+  c[0] = node % (nx + 1);
+  int t = node / (nx + 1);
+  if (DIM == 2) {
+    c[1] = t;
+    c[2] = 0;
+  } else {
+    c[1] = t % (ny + 1);
+    c[2] = t / (ny + 1);
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ int node_of(const int* c, int nx, int ny) {
+  return (DIM == 2) ? c[0] + c[1] * (nx + 1) : c[0] + c[1] * (nx + 1) + c[2] * (nx + 1) * (ny + 1);
+}
+
+// block index of window bit `bit` in a row, or -1 when the block is absent
+__device__ __forceinline__ int block_at(const int* rp, const uint64_t* mask, int row, int bit) {
+  uint64_t m0 = mask[2 * row], m1 = mask[2 * row + 1];
+  if (bit < 64) {
+    if (!((m0 >> bit) & 1ull)) return -1;
+    return rp[row] + __popcll(m0 & ((1ull << bit) - 1ull));
+  }
+  int b = bit - 64;
+  if (!((m1 >> b) & 1ull)) return -1;
+  return rp[row] + __popcll(m0) + __popcll(m1 & ((1ull << b) - 1ull));
+}
+
+struct Dims {
+  int n[3];   // cells per axis
+};
+
+template <int DIM, int NB>
+__global__ void k_free(Dims f, Dims c, int n_coarse, const uint8_t* freem_f, uint8_t* freem_c) {
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n_coarse) return;
+  int A[3];
+  coords<DIM>(I, c.n[0], c.n[1], A);
+  uint8_t fr[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) fr[k] = 0;
+  const int oz = (DIM == 3) ? 1 : 0;
+  for (int dz = -oz; dz <= oz; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int a[3] = {2 * A[0] + dx, 2 * A[1] + dy, 2 * A[2] + dz};
+        if (a[0] < 0 || a[0] > f.n[0] || a[1] < 0 || a[1] > f.n[1]) continue;
+        if (DIM == 3 && (a[2] < 0 || a[2] > f.n[2])) continue;
+        int i = node_of<DIM>(a, f.n[0], f.n[1]);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) fr[k] |= freem_f[(long)i * NB + k];
+      }
+#pragma unroll
+  for (int k = 0; k < NB; ++k) freem_c[(long)I * NB + k] = fr[k] ? 1 : 0;
+}
+
+// AP[i, t] = sum_j A[i, j] P[j, J(t)] (free columns only); t indexes the 4^DIM coarse
+// nodes J with J_axis = floor((i_axis - 2) / 2) + t_axis.
+template <int DIM, int NB, typename T>
+__global__ void k_ap(Dims f, Dims c, int n_fine, const int* rp, const uint64_t* mask, const T* vals,
+                     const uint8_t* freem_f, T* AP) {
+  const int TT = (DIM == 2) ? 16 : 64;
+  long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (gid >= (long)n_fine * TT) return;
+  int i = (int)(gid / TT), t = (int)(gid % TT);
+  int a[3];
+  coords<DIM>(i, f.n[0], f.n[1], a);
+  int tt[3] = {t & 3, (t >> 2) & 3, (t >> 4) & 3};
+  int J[3] = {0, 0, 0};
+  bool ok = true;
+#pragma unroll
+  for (int ax = 0; ax < DIM; ++ax) {
+    J[ax] = floordiv2(a[ax] - 2) + tt[ax];
+    if (J[ax] < 0 || J[ax] > c.n[ax]) ok = false;
+  }
+  double acc[NB * NB];
+#pragma unroll
+  for (int k = 0; k < NB * NB; ++k) acc[k] = 0.0;
+  if (ok) {
+    const int oz = (DIM == 3) ? 1 : 0;
+    for (int dz = -oz; dz <= oz; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          int j[3] = {2 * J[0] + dx, 2 * J[1] + dy, 2 * J[2] + dz};
+          int o[3] = {j[0] - a[0], j[1] - a[1], j[2] - a[2]};
+          bool in = true;
+#pragma unroll
+          for (int ax = 0; ax < DIM; ++ax)
+            if (j[ax] < 0 || j[ax] > f.n[ax] || o[ax] < -2 || o[ax] > 2) in = false;
+          if (!in) continue;
+          int bit = (o[0] + 2) + 5 * (o[1] + 2) + ((DIM == 3) ? 25 * (o[2] + 2) : 0);
+          int q = block_at(rp, mask, i, bit);
+          if (q < 0) continue;
+          double w = (dx == 0 ? 1.0 : 0.5) * (dy == 0 ? 1.0 : 0.5) * ((DIM == 3 && dz != 0) ? 0.5 : 1.0);
+          int jn = node_of<DIM>(j, f.n[0], f.n[1]);
+          const T* blk = vals + (long)q * NB * NB;
+#pragma unroll
+          for (int cc = 0; cc < NB; ++cc) {
+            if (!freem_f[(long)jn * NB + cc]) continue;
+#pragma unroll
+            for (int r = 0; r < NB; ++r) acc[r * NB + cc] += w * (double)blk[r * NB + cc];
+          }
+        }
+  }
+  T* out = AP + gid * NB * NB;
+#pragma unroll
+  for (int k = 0; k < NB * NB; ++k) out[k] = (T)acc[k];
+}
+
+// A_c[I, J] = sum_i P[i, I] AP[i, J] over free fine rows; identity on coarse dofs without free fine dofs.
+template <int DIM, int NB, typename T>
+__global__ void k_rap(Dims f, Dims c, int n_coarse, const int* rp_c, const uint64_t* mask_c, T* vals_c,
+                      const uint8_t* freem_f, const uint8_t* freem_c, const T* AP) {
+  const int TT = (DIM == 2) ? 16 : 64;
+  const int W = (DIM == 2) ? 25 : 125;
+  long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (gid >= (long)n_coarse * W) return;
+  int I = (int)(gid / W), s = (int)(gid % W);
+  int q = block_at(rp_c, mask_c, I, s);
+  if (q < 0) return;
+  int A[3];
+  coords<DIM>(I, c.n[0], c.n[1], A);
+  int so[3] = {s % 5 - 2, (s / 5) % 5 - 2, (DIM == 3) ? s / 25 - 2 : 0};
+  int J[3] = {A[0] + so[0], A[1] + so[1], A[2] + so[2]};
+  double acc[NB * NB];
+#pragma unroll
+  for (int k = 0; k < NB * NB; ++k) acc[k] = 0.0;
+  const int oz = (DIM == 3) ? 1 : 0;
+  for (int dz = -oz; dz <= oz; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int a[3] = {2 * A[0] + dx, 2 * A[1] + dy, 2 * A[2] + dz};
+        bool in = true;
+        int t = 0, mul = 1;
+#pragma unroll
+        for (int ax = 0; ax < DIM; ++ax) {
+          if (a[ax] < 0 || a[ax] > f.n[ax]) in = false;
+          int ta = J[ax] - floordiv2(a[ax] - 2);
+          if (ta < 0 || ta > 3) in = false;
+          t += ta * mul;
+          mul *= 4;
+        }
+        if (!in) continue;
+        double w = (dx == 0 ? 1.0 : 0.5) * (dy == 0 ? 1.0 : 0.5) * ((DIM == 3 && dz != 0) ? 0.5 : 1.0);
+        int i = node_of<DIM>(a, f.n[0], f.n[1]);
+        const T* blk = AP + ((long)i * TT + t) * NB * NB;
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+          if (!freem_f[(long)i * NB + r]) continue;
+#pragma unroll
+          for (int cc = 0; cc < NB; ++cc) acc[r * NB + cc] += w * (double)blk[r * NB + cc];
+        }
+      }
+  if (so[0] == 0 && so[1] == 0 && so[2] == 0)
+#pragma unroll
+    for (int r = 0; r < NB; ++r)
+      if (!freem_c[(long)I * NB + r]) acc[r * NB + r] = 1.0;
+  T* out = vals_c + (long)q * NB * NB;
+#pragma unroll
+  for (int k = 0; k < NB * NB; ++k) out[k] = (T)acc[k];
+}
+
+// b_c = P^T r_f (free fine dofs only)
+template <int DIM, int NB, typename T>
+__global__ void k_restrict(Dims f, Dims c, int n_coarse, const uint8_t* freem_f, const T* r_f, T* b_c) {
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n_coarse) return;
+  int A[3];
+  coords<DIM>(I, c.n[0], c.n[1], A);
+  double acc[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) acc[k] = 0.0;
+  const int oz = (DIM == 3) ? 1 : 0;
+  for (int dz = -oz; dz <= oz; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int a[3] = {2 * A[0] + dx, 2 * A[1] + dy, 2 * A[2] + dz};
+        if (a[0] < 0 || a[0] > f.n[0] || a[1] < 0 || a[1] > f.n[1]) continue;
+        if (DIM == 3 && (a[2] < 0 || a[2] > f.n[2])) continue;
+        double w = (dx == 0 ? 1.0 : 0.5) * (dy == 0 ? 1.0 : 0.5) * ((DIM == 3 && dz != 0) ? 0.5 : 1.0);
+        int i = node_of<DIM>(a, f.n[0], f.n[1]);
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+          if (freem_f[(long)i * NB + k]) acc[k] += w * (double)r_f[(long)i * NB + k];
+      }
+#pragma unroll
+  for (int k = 0; k < NB; ++k) b_c[(long)I * NB + k] = (T)acc[k];
+}
+
+// x_f += P x_c (free fine dofs only)
+template <int DIM, int NB, typename T>
+__global__ void k_prolong_add(Dims f, Dims c, int n_fine, const uint8_t* freem_f, const T* x_c, T* x_f) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_fine) return;
+  int a[3];
+  coords<DIM>(i, f.n[0], f.n[1], a);
+  int lo[3], hi[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    lo[ax] = a[ax] >> 1;
+    hi[ax] = (a[ax] + 1) >> 1;  // == lo for even a
+  }
+  double acc[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) acc[k] = 0.0;
+  const int zc = (DIM == 3) ? 2 : 1;
+  for (int pz = 0; pz < zc; ++pz)
+    for (int py = 0; py < 2; ++py)
+      for (int px = 0; px < 2; ++px) {
+        int A[3] = {px ? hi[0] : lo[0], py ? hi[1] : lo[1], pz ? hi[2] : lo[2]};
+        // even coordinate: one parent (skip the duplicate), odd: two parents of weight 0.5
+        if ((px && hi[0] == lo[0]) || (py && hi[1] == lo[1]) || (DIM == 3 && pz && hi[2] == lo[2])) continue;
+        double w = ((a[0] & 1) ? 0.5 : 1.0) * ((a[1] & 1) ? 0.5 : 1.0) * ((DIM == 3 && (a[2] & 1)) ? 0.5 : 1.0);
+        int J = node_of<DIM>(A, c.n[0], c.n[1]);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) acc[k] += w * (double)x_c[(long)J * NB + k];
+      }
+#pragma unroll
+  for (int k = 0; k < NB; ++k)
+    if (freem_f[(long)i * NB + k]) x_f[(long)i * NB + k] = (T)((double)x_f[(long)i * NB + k] + acc[k]);
+}
+
+// x = omega Dinv b
+template <int NB, typename T>
+__global__ void k_jac_first(int n_rows, const T* dinv, const T* b, double omega, T* x) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n_rows) return;
+  double v[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) v[j] = (double)b[(long)row * NB + j];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) s += (double)dinv[(long)row * NB * NB + r * NB + j] * v[j];
+    x[(long)row * NB + r] = (T)(omega * s);
+  }
+}
+
+// x += omega Dinv (b - Ax)
+template <int NB, typename T>
+__global__ void k_jac_update(int n_rows, const T* dinv, const T* b, const T* ax, double omega, T* x) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n_rows) return;
+  double v[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) v[j] = (double)b[(long)row * NB + j] - (double)ax[(long)row * NB + j];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) s += (double)dinv[(long)row * NB * NB + r * NB + j] * v[j];
+    x[(long)row * NB + r] = (T)((double)x[(long)row * NB + r] + omega * s);
+  }
+}
+
+template <typename T>
+__global__ void k_axpy_t(long n, const T* a, T* x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = (T)((double)x[i] + (double)a[i]);
+}
+
+template <typename T>
+__global__ void k_sub_inplace(long n, const T* b, T* r) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    r[i] = (T)((double)b[i] - (double)r[i]);
+}
+
+inline Dims dims_of(const gm_mg_level& l) {
+  Dims d;
+  d.n[0] = l.nx;
+  d.n[1] = l.ny;
+  d.n[2] = l.nz;
+  return d;
+}
+
+struct Cycle {
+  int dim, n_levels, nu_pre, nu_post, coarse_sweeps;
+  double omega;
+  const gm_mg_level* L;
+};
+
+template <int DIM, int NB, typename T>
+static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
+  const gm_mg_level& lv = cy.L[l];
+  int n = lv.n_nodes;
+  int rb = gm_blocks(n, 256);
+  const T* dinv = (const T*)lv.dinv;
+  const T* vals = (const T*)lv.vals;
+  T* r = (T*)lv.r;
+  bool coarsest = (l == cy.n_levels - 1);
+  int sweeps = coarsest ? cy.coarse_sweeps : cy.nu_pre;
+  k_jac_first<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, cy.omega, x);
+  for (int k = 1; k < sweeps; ++k) {
+    launch_spmv<NB, T, T, T>(n, lv.row_ptr, lv.col_idx, vals, x, r, nullptr, nullptr, nullptr, st);
+    k_jac_update<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, r, cy.omega, x);
+  }
+  GM_LAUNCH_CHECK();
+  if (coarsest) return 0;
+  const gm_mg_level& cv = cy.L[l + 1];
+  launch_spmv<NB, T, T, T>(n, lv.row_ptr, lv.col_idx, vals, x, r, nullptr, nullptr, nullptr, st);
+  long nv = (long)n * NB;
+  k_sub_inplace<T><<<dot_blocks(nv), RED_T, 0, st>>>(nv, b, r);
+  k_restrict<DIM, NB, T><<<gm_blocks(cv.n_nodes, 256), 256, 0, st>>>(dims_of(lv), dims_of(cv), cv.n_nodes, lv.freem,
+                                                                      r, (T*)cv.b);
+  GM_LAUNCH_CHECK();
+  int rc = vcycle<DIM, NB, T>(cy, l + 1, (const T*)cv.b, (T*)cv.x, st);
+  if (rc) return rc;
+  k_prolong_add<DIM, NB, T><<<rb, 256, 0, st>>>(dims_of(lv), dims_of(cv), n, lv.freem, (const T*)cv.x, x);
+  for (int k = 0; k < cy.nu_post; ++k) {
+    launch_spmv<NB, T, T, T>(n, lv.row_ptr, lv.col_idx, vals, x, r, nullptr, nullptr, nullptr, st);
+    k_jac_update<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, r, cy.omega, x);
+  }
+  GM_LAUNCH_CHECK();
+  return 0;
+}
+
+template <int DIM, int NB, typename T>
+static int setup(int n_levels, gm_mg_level* L, void* ap_work, hipStream_t st) {
+  for (int l = 0; l + 1 < n_levels; ++l) {
+    gm_mg_level& f = L[l];
+    gm_mg_level& c = L[l + 1];
+    Dims df = dims_of(f), dc = dims_of(c);
+    k_free<DIM, NB><<<gm_blocks(c.n_nodes, 256), 256, 0, st>>>(df, dc, c.n_nodes, f.freem, c.freem);
+    const int TT = (DIM == 2) ? 16 : 64;
+    const int W = (DIM == 2) ? 25 : 125;
+    long nap = (long)f.n_nodes * TT;
+    k_ap<DIM, NB, T><<<gm_blocks(nap, 256), 256, 0, st>>>(df, dc, f.n_nodes, f.row_ptr, f.row_mask,
+                                                           (const T*)f.vals, f.freem, (T*)ap_work);
+    long nrap = (long)c.n_nodes * W;
+    k_rap<DIM, NB, T><<<gm_blocks(nrap, 256), 256, 0, st>>>(df, dc, c.n_nodes, c.row_ptr, c.row_mask, (T*)c.vals,
+                                                             f.freem, c.freem, (const T*)ap_work);
+    GM_LAUNCH_CHECK();
+  }
+  for (int l = 0; l < n_levels; ++l)
+    k_diag_inv<NB, T><<<gm_blocks(L[l].n_nodes, 256), 256, 0, st>>>(L[l].n_nodes, L[l].row_ptr, L[l].col_idx,
+                                                                     (const T*)L[l].vals, (T*)L[l].dinv);
+  GM_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace mg
+
+static int mg_check(int dim, int nb, int n_levels, const gm_mg_level* levels) {
+  GM_REQUIRE(levels && n_levels >= 1 && n_levels <= 16, "multigrid: bad level list");
+  GM_REQUIRE(dim == 2 || dim == 3, "multigrid: dim must be 2 or 3");
+  GM_REQUIRE(nb >= 1 && nb <= 4, "multigrid: nb must be 1..4");
+  for (int l = 0; l < n_levels; ++l) {
+    const gm_mg_level& v = levels[l];
+    GM_REQUIRE(v.row_ptr && v.col_idx && v.row_mask && v.vals && v.dinv && v.freem && v.r,
+               "multigrid: level array missing");
+    long expect = (long)(v.nx + 1) * (v.ny + 1) * (dim == 3 ? (v.nz + 1) : 1);
+    GM_REQUIRE(expect == v.n_nodes, "multigrid: n_nodes does not match the level's grid");
+    if (l > 0) {
+      const gm_mg_level& f = levels[l - 1];
+      GM_REQUIRE(v.x && v.b, "multigrid: coarse level vectors missing");
+      GM_REQUIRE(v.nx == (f.nx + 1) / 2 && v.ny == (f.ny + 1) / 2 && (dim == 2 || v.nz == (f.nz + 1) / 2),
+                 "multigrid: coarse grid is not ceil(n/2) of the finer one");
+    }
+  }
+  return 0;
+}
+
+#define GM_MG_DISPATCH(CALL)                                  \
+  do {                                                        \
+    if (dim == 2) {                                           \
+      if (nb == 1) CALL(2, 1);                                \
+      if (nb == 2) CALL(2, 2);                                \
+      if (nb == 3) CALL(2, 3);                                \
+      if (nb == 4) CALL(2, 4);                                \
+    } else {                                                  \
+      if (nb == 1) CALL(3, 1);                                \
+      if (nb == 2) CALL(3, 2);                                \
+      if (nb == 3) CALL(3, 3);                                \
+      if (nb == 4) CALL(3, 4);                                \
+    }                                                         \
+  } while (0)
+
+extern "C" int gm_mg_setup(int dim, int nb, int fp32, int n_levels, gm_mg_level* levels, void* ap_work,
+                           hipStream_t s) {
+  int rc = mg_check(dim, nb, n_levels, levels);
+  if (rc) return rc;
+  GM_REQUIRE(n_levels == 1 || ap_work, "gm_mg_setup: missing AP workspace");
+#define GM_SETUP(D, B)                                                                      \
+  return fp32 ? mg::setup<D, B, float>(n_levels, levels, ap_work, s)                       \
+              : mg::setup<D, B, double>(n_levels, levels, ap_work, s)
+  GM_MG_DISPATCH(GM_SETUP);
+#undef GM_SETUP
+  return -1;
+}
+
+extern "C" int gm_mg_vcycle(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre,
+                            int nu_post, int coarse_sweeps, double omega, const void* b, void* x, hipStream_t s) {
+  int rc = mg_check(dim, nb, n_levels, levels);
+  if (rc) return rc;
+  GM_REQUIRE(b && x && nu_pre >= 1 && nu_post >= 0 && coarse_sweeps >= 1, "gm_mg_vcycle: bad argument");
+  mg::Cycle cy{dim, n_levels, nu_pre, nu_post, coarse_sweeps, omega, levels};
+#define GM_CYC(D, B)                                                                          \
+  return fp32 ? mg::vcycle<D, B, float>(cy, 0, (const float*)b, (float*)x, s)                \
+              : mg::vcycle<D, B, double>(cy, 0, (const double*)b, (double*)x, s)
+  GM_MG_DISPATCH(GM_CYC);
+#undef GM_CYC
+  return -1;
+}
+
+// ---- GMRES with the multigrid V-cycle as right preconditioner
+namespace mg {
+template <int DIM, int NB, typename T>
+struct MGPrec {
+  Cycle cy;
+  long n;
+  int apply(const T* in, T* out, hipStream_t st) const { return vcycle<DIM, NB, T>(cy, 0, in, out, st); }
+  int apply_add(const T* in, T* x, T* tmp, hipStream_t st) const {
+    int rc = vcycle<DIM, NB, T>(cy, 0, in, tmp, st);
+    if (rc) return rc;
+    k_axpy_t<T><<<dot_blocks(n), RED_T, 0, st>>>(n, tmp, x);
+    GM_LAUNCH_CHECK();
+    return 0;
+  }
+};
+}  // namespace mg
+
+extern "C" int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre,
+                           int nu_post, int coarse_sweeps, double omega, const void* b, void* x, void* V, void* z,
+                           void* w, double* part, double* dh, int m, double tol, int maxiter, int* iters,
+                           double* relres, hipStream_t s) {
+  int rc = mg_check(dim, nb, n_levels, levels);
+  if (rc) return rc;
+  GM_REQUIRE(iters && relres && m >= 1 && m <= 256 && b && x && V && z && w, "gm_gmres_mg: bad argument");
+  GM_REQUIRE(nu_pre >= 1 && nu_post >= 0 && coarse_sweeps >= 1, "gm_gmres_mg: bad smoother counts");
+  mg::Cycle cy{dim, n_levels, nu_pre, nu_post, coarse_sweeps, omega, levels};
+  const gm_mg_level& l0 = levels[0];
+  long n = (long)l0.n_nodes * nb;
+#define GM_GMG(D, B)                                                                                            \
+  do {                                                                                                          \
+    if (fp32) {                                                                                                 \
+      mg::MGPrec<D, B, float> pc{cy, n};                                                                        \
+      return gmres_impl<B, float, float>(l0.n_nodes, l0.row_ptr, l0.col_idx, (const float*)l0.vals,             \
+                                         (const float*)b, (float*)x, (float*)V, (float*)z, (float*)w, pc, part, \
+                                         dh, m, tol, maxiter, iters, relres, s);                                \
+    }                                                                                                           \
+    mg::MGPrec<D, B, double> pc{cy, n};                                                                         \
+    return gmres_impl<B, double, double>(l0.n_nodes, l0.row_ptr, l0.col_idx, (const double*)l0.vals,            \
+                                         (const double*)b, (double*)x, (double*)V, (double*)z, (double*)w, pc,  \
+                                         part, dh, m, tol, maxiter, iters, relres, s);                          \
+  } while (0)
+  GM_MG_DISPATCH(GM_GMG);
+#undef GM_GMG
   return -1;
 }

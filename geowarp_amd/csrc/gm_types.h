@@ -128,6 +128,24 @@ typedef struct gm_local_args {
   const double* abar;      /* n (optional) */
 } gm_local_args;
 
+/* One level of the geometric multigrid preconditioner on the background grid
+   (level 0 = the Jacobian's own grid and pattern; level l+1 has every second
+   node of level l per axis, ceil(n/2) cells).  Patterns are block-CSR with a
+   5^dim window bitmask per row (row_mask, 2 words per row).  vals/dinv and the
+   vectors x, b, r hold the solve precision (double or float). */
+typedef struct gm_mg_level {
+  int nx, ny, nz;       /* cells per axis (nz = 0 in 2D) */
+  int n_nodes;
+  int nnzb;
+  const int* row_ptr;
+  const int* col_idx;
+  const uint64_t* row_mask;
+  void* vals;           /* nnzb * nb * nb */
+  void* dinv;           /* n_nodes * nb * nb */
+  uint8_t* freem;       /* n_nodes * nb: 1 = free dof */
+  void* x; void* b; void* r;  /* n_nodes * nb work vectors (level 0 x/b unused) */
+} gm_mg_level;
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@ import time
 import torch
 
 from .. import _lib as L
+from ._multigrid import Multigrid
 
 MAT_KIND = {'Hencky elasticity': 0, 'J2': 1, 'Neo-Hookean': 2, 'Drucker-Prager': 3}
 
@@ -89,6 +90,8 @@ class Engine:
         self.n_particles = -1
         self.timers = None
         self.stats = dict(krylov_iters=[], krylov_relres=[])
+        self.jac_version = 0
+        self.mg_options = {}
 
     # ------------------------------------------------------------------ buffers
     def _alloc_vals(self):
@@ -244,8 +247,15 @@ class Engine:
         L.check(L.lib().gm_norm2(v.numel(), L.ptr(v), L.ptr(self.part), L.c_void_p(self.scal.data_ptr() + 8 * slot),
                                  _stream()), 'gm_norm2')
 
+    def multigrid(self, fp32):
+        key = ('mg', bool(fp32))
+        if key not in self._ws:
+            self._ws[key] = [Multigrid(self, fp32, **self.mg_options), -1]
+        return self._ws[key]
+
     def jacobian(self, x, lp):
         fp32 = self.precision == 'mixed'
+        self.jac_version += 1
         L.check(L.lib().gm_node_jacobian(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins),
                                          L.ctypes.byref(self.pattern), L.ptr(self.D), int(fp32), L.ptr(self.freem),
                                          L.ptr(self.vals), int(fp32), int(self.coupled), _stream()),
@@ -266,11 +276,20 @@ class Engine:
         return self._ws[key]
 
     def krylov(self, b, x, fp32, tol, maxiter, method='gmres', restart=40, check_every=8):
-        """One Krylov solve K x = b on the current Jacobian (block-Jacobi
-        preconditioned GMRES(restart) or BiCGStab); returns (iterations, relres)."""
+        """One Krylov solve K x = b on the current Jacobian: GMRES(restart)
+        preconditioned by geometric-multigrid V-cycles ('mg') or block Jacobi
+        ('gmres'), or block-Jacobi BiCGStab ('bicgstab'); returns (iterations, relres)."""
         lib = L.lib()
         it = L.c_int(0)
         rr = L.c_double(0.0)
+        if method == 'mg':
+            entry = self.multigrid(fp32)
+            mgh = entry[0]
+            if entry[1] != self.jac_version:
+                mgh.setup(_stream())
+                entry[1] = self.jac_version
+            V, z, w, dinv, part, dh = self.gmres_workspace(self.nb, fp32, restart)
+            return mgh.gmres(b, x, V, z, w, part, dh, restart, tol, maxiter, _stream())
         if method == 'gmres':
             V, z, w, dinv, part, dh = self.gmres_workspace(self.nb, fp32, restart)
             L.check(lib.gm_gmres(self.nb, int(fp32), self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),

@@ -213,6 +213,33 @@ int gm_local_vjp_2d(const gm_local_args* args, double* g, hipStream_t stream);
 int gm_bsr_transpose(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
                      void* vals_t, hipStream_t stream);
 
+/* ---------------------------------------------------------------- geometric multigrid
+   Multilevel preconditioner for the Newton systems.  The reference's multilevel
+   option is pyamg smoothed aggregation + BiCGStab on the host
+   (qs2d:205-213, qc2d:288-293); here: geometric coarsening of the background
+   grid (every second node per axis), trilinear prolongation restricted to free
+   dofs, Galerkin coarse operators P^T A P built on the device, damped
+   block-Jacobi smoothing. */
+
+/* Build levels 1..n_levels-1 from level 0 (whose pattern, values and free mask
+   are the Jacobian's) and every level's block-diagonal inverse.  Coarse
+   patterns (row_ptr, col_idx, row_mask: full 5^dim windows clipped to the
+   grid) and all buffers are the caller's.  ap_work: level-0 n_nodes * 4^dim *
+   nb * nb values of the solve precision. */
+int gm_mg_setup(int dim, int nb, int fp32, int n_levels, gm_mg_level* levels, void* ap_work, hipStream_t stream);
+
+/* x = M^-1 b for one V-cycle (nu_pre / nu_post damped block-Jacobi sweeps with
+   factor omega, coarse_sweeps sweeps on the coarsest level). */
+int gm_mg_vcycle(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre, int nu_post,
+                 int coarse_sweeps, double omega, const void* b, void* x, hipStream_t stream);
+
+/* GMRES(m) on level 0's matrix, right-preconditioned by the V-cycle; arguments
+   as gm_gmres (V: (m+1) * n vectors, z, w: n; part >= 64(m+1) and >= 2048
+   doubles; dh >= 3(m+2) doubles). */
+int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre, int nu_post,
+                int coarse_sweeps, double omega, const void* b, void* x, void* V, void* z, void* w, double* part,
+                double* dh, int m, double tol, int maxiter, int* iters, double* relres, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
