@@ -43,6 +43,9 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--verbose', action='store_true')
+    ap.add_argument('--adjoint-steps', type=int, default=5,
+                    help='load steps of the batched-specimen adjoint measurement (0 = skip)')
+    ap.add_argument('--krylov', type=str, default='mg', help="'mg' (multigrid GMRES) or 'gmres' (block Jacobi)")
     return ap.parse_args()
 
 
@@ -50,8 +53,9 @@ def build_solver(cells, precision, material, **kw):
     import geowarp_amd.solvers as gs
     from geowarp_amd import problems as P
     dx = 1.0 / 64
+    kw.setdefault('n_iter', 12)
     cfg = P.indent3d(n_cells=cells, dx=dx, total_steps=40, depth=0.05, footing=0.2 * cells[0] / 64.0,
-                     material=material, precision=precision, tol=1e-8, n_iter=12, **kw)
+                     material=material, precision=precision, tol=1e-8, **kw)
     s = P.build_solver(gs, cfg, device='cuda:0')
     cfg['particle_flags'](s.x_particles, s.particle_traction_flag_array, s.particle_pressure_boundary_flag_array)
     return s, cfg
@@ -102,6 +106,47 @@ def spmv_roofline(s, reps=50):
     achieved = bytes_ / (avg_ms * 1e-3) / 1e9
     return dict(kernel='k_spmv<%d,8,%s>' % (nb, 'float' if fp32 else 'double'), avg_us=avg_ms * 1e3,
                 bytes_per_launch=bytes_, nnzb=eng.nnzb, achieved_GBs=achieved)
+
+
+def adjoint_bench(steps, total=30, every=5, n_specimens=64, krylov='mg'):
+    """Gradient of a least-squares reaction-force misfit with respect to every
+    specimen's friction angle and cohesion (2 x 64 components) through the
+    hand-written adjoint: synthetic force curves from the FP64 forward at the
+    true parameters (untimed), then one timed forward + adjoint sweep at a trial
+    parameter set."""
+    import torch
+    import geowarp_amd.solvers as gs
+    from geowarp_amd import problems as P
+    from geowarp_amd.adjoint import LoadPathAdjoint, LoadPathFunction
+    W, phi, coh = P.specimen_counts(n_specimens, seed=0)
+    cfg = P.specimens2d(W, phi, coh, total_steps=total, strain=0.03, tol=1e-8, n_iter=12)
+    cfg['iteration_dict']['krylov'] = krylov
+    s = P.build_solver(gs, cfg, device='cuda:0')
+    cfg['set_boundary_dofs'](s.dofStruct.boundary_flag_array, s.n_grid_x, s.n_nodes)
+    dev = s.device
+    g = torch.as_tensor(cfg['groups'], device=dev)
+    adj = LoadPathAdjoint(s, total, n_steps=steps, groups=g, checkpoint_every=every)
+    with torch.no_grad():
+        f_obs = LoadPathFunction.apply(torch.tensor(phi, device=dev), torch.tensor(coh, device=dev), adj)
+    phi_t = torch.tensor(phi + 2.0, device=dev, requires_grad=True)
+    coh_t = torch.tensor(coh + 3.0, device=dev, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f = LoadPathFunction.apply(phi_t, coh_t, adj)
+    loss = ((f - f_obs) ** 2).sum()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return dict(workload='%d 2D Drucker-Prager specimens (seed 0, %d particles, %d-%d cells wide), load steps '
+                         '0-%d of %d, checkpoint every %d, loss = sum of squared reaction-force misfits'
+                         % (n_specimens, s.n_particles, int(W.min()), int(W.max()), steps - 1, total, every),
+                seconds_forward=round(t1 - t0, 3), seconds_adjoint=round(t2 - t1, 3),
+                grads_per_s=round(1.0 / (t2 - t0), 4), gradient_components=2 * n_specimens,
+                forward_steps=adj.stats['forward_steps'], adjoint_steps=adj.stats['adjoint_steps'],
+                transposed_solve_iters=adj.stats['krylov_iters'],
+                grad_norm=dict(phi=float(phi_t.grad.norm()), coh=float(coh_t.grad.norm())))
 
 
 def field_errors(a, b):
@@ -164,7 +209,7 @@ def main():
     forms = ['mixed'] if args.no_fp64 else ['fp64', 'mixed']
     solvers = {}
     for form in forms:
-        s, cfg = build_solver(cells, form, args.material)
+        s, cfg = build_solver(cells, form, args.material, krylov=args.krylov)
         solvers[form] = s
         run_steps(s, cfg, 0, args.warmup)
         s.engine.timers = PhaseTimer()
@@ -215,6 +260,8 @@ def main():
     if 'fp64' in res:
         out['mixed_vs_fp64_speedup'] = round(res['fp64']['seconds'] / res['mixed']['seconds'], 3)
         out['mixed_vs_fp64_field_rel_err'] = field_errors(solvers['mixed'], solvers['fp64'])
+    if args.adjoint_steps > 0:
+        out['adjoint'] = adjoint_bench(args.adjoint_steps, krylov=args.krylov)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline()
     print(json.dumps(out))

@@ -64,7 +64,8 @@ class gm_local_args(ctypes.Structure):
 class gm_mg_level(ctypes.Structure):
     _fields_ = [('nx', c_int), ('ny', c_int), ('nz', c_int), ('n_nodes', c_int), ('nnzb', c_int),
                 ('row_ptr', c_void_p), ('col_idx', c_void_p), ('row_mask', c_void_p), ('vals', c_void_p),
-                ('dinv', c_void_p), ('freem', c_void_p), ('x', c_void_p), ('b', c_void_p), ('r', c_void_p)]
+                ('dinv', c_void_p), ('freem', c_void_p), ('x', c_void_p), ('b', c_void_p), ('r', c_void_p),
+                ('dense_inv', c_void_p)]
 
 
 class gm_krylov_ws(ctypes.Structure):
@@ -113,7 +114,10 @@ _SIGS = {
     'gm_mg_setup': (c_int, [c_int, c_int, c_int, c_int, _P, _P, _P]),
     'gm_mg_vcycle': (c_int, [c_int, c_int, c_int, c_int, _P, c_int, c_int, c_int, c_double, _P, _P, _P]),
     'gm_gmres_mg': (c_int, [c_int, c_int, c_int, c_int, _P, c_int, c_int, c_int, c_double, _P, _P, _P, _P, _P,
-                            _P, _P, c_int, c_double, c_int, _P, _P, _P]),
+                            _P, _P, _P, c_int, c_double, c_int, _P, _P, _P]),
+    'gm_up_setup': (c_int, [c_int, c_int, c_int, c_int, c_int, _P, _P, _P, _P, c_int, _P, _P, _P, _P]),
+    'gm_gmres_up': (c_int, [c_int, c_int, c_int, c_int, _P, _P, _P, _P, c_int, _P, c_int, c_int, c_int, c_double,
+                            _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, c_int, c_double, c_int, _P, _P, _P]),
 }
 
 _lib = None

@@ -19,7 +19,8 @@ state cotangent Sbar_{k+1} carried back from step k+1:
      terms of Sbar_k and of the parameters;
   2. ubar = gather over particles of Hbar and of xbar_{k+1} (the transposes of
      the gradient and the G2P interpolation), masked to free dofs;
-  3. transposed solve K^T lambda = ubar (block transpose of K, GMRES);
+  3. transposed solve K^T lambda = ubar (block transpose of K; GMRES with the
+     solver's preconditioner: multigrid on K^T or block Jacobi);
   4. local VJP with the residual cotangent lambda: Sbar_k and parameter terms
      through R (du/dS = K^-1 dR/dS).
 
@@ -185,12 +186,15 @@ class LoadPathAdjoint:
                                          L.ptr(eng.vals), L.ptr(vt), _stream()), 'gm_bsr_transpose')
         vals = eng.vals
         eng.vals = vt
+        eng.jac_version += 1  # a multigrid hierarchy is rebuilt for K^T (and again for K afterwards)
         lam = torch.zeros_like(ubar)
+        method = 'mg' if self.s.krylov_method == 'mg' else 'gmres'
         try:
-            its, rel = eng.krylov(ubar, lam, False, self.adjoint_tol, self.s.linear_maxiter, 'gmres',
+            its, rel = eng.krylov(ubar, lam, False, self.adjoint_tol, self.s.linear_maxiter, method,
                                   self.s.krylov_restart)
         finally:
             eng.vals = vals
+            eng.jac_version += 1
         self.stats['krylov_iters'].append(its)
         return lam
 

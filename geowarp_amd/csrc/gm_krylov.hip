@@ -314,6 +314,25 @@ __global__ void k_bicg_init_sc(double* sc) {
   sc[S_RR] = sc[S_BB];
 }
 
+// restart from the current x: r = rhat = b - Ax (ax holds Ax), p = v = 0
+template <typename TV>
+__global__ void k_bicg_restart_vecs(long n, const TV* b, const TV* ax, TV* r, TV* rhat, TV* p, TV* v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    TV rv = (TV)((double)b[i] - (double)ax[i]);
+    r[i] = rv;
+    rhat[i] = rv;
+    p[i] = (TV)0;
+    v[i] = (TV)0;
+  }
+}
+__global__ void k_bicg_restart_sc(double* sc) {
+  sc[S_ALPHA] = 1.0;
+  sc[S_OMEGA] = 1.0;
+  sc[S_BETA] = 0.0;
+  sc[S_FLAG] = 0.0;
+  sc[S_RHO] = sc[S_RR];
+}
+
 template <typename TV>
 __global__ void k_copy_zero(long n, const TV* b, TV* r, TV* rhat, TV* p, TV* v, TV* x) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -360,7 +379,7 @@ static int bicgstab_impl(int n_rows, const int* rp, const int* ci, const TM* val
     }
     return 0;
   }
-  int it = 0;
+  int it = 0, restarts = 0;
   double rel = 1.0;
   while (it < maxiter) {
     int nrun = check_every;
@@ -389,7 +408,17 @@ static int bicgstab_impl(int n_rows, const int* rp, const int* ci, const TM* val
       *relres_out = rel;
       return -5;
     }
-    if (rel <= tol || hs[S_FLAG] != 0.0) break;
+    if (rel <= tol) break;
+    if (hs[S_FLAG] != 0.0) {
+      // breakdown (rho, (rhat, v) or omega vanished): restart from the current iterate
+      if (++restarts > 64) break;
+      launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, x, t, nullptr, nullptr, nullptr, st);
+      k_bicg_restart_vecs<TV><<<db, RED_T, 0, st>>>(n, b, t, r, rhat, p, v);
+      k_dot2<TV><<<db, RED_T, 0, st>>>(n, r, r, nullptr, nullptr, part);
+      k_sum_parts<<<1, RED_T, 0, st>>>(part, db, sc + S_RR, nullptr);
+      k_bicg_restart_sc<<<1, 1, 0, st>>>(sc);
+      GM_LAUNCH_CHECK();
+    }
   }
   *iters_out = it;
   *relres_out = rel;
@@ -633,6 +662,12 @@ __global__ void k_apply_dinv(int n_rows, const TM* dinv, const TV* in, TV* out) 
   apply_dinv<NB, TM, TV>(dinv, row, in, out);
 }
 
+template <typename TV>
+__global__ void k_add_into(long n, const TV* a, TV* x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = (TV)((double)x[i] + (double)a[i]);
+}
+
 // Block-Jacobi right preconditioner: out = Dinv in; x += Dinv in.
 template <int NB, typename TM, typename TV>
 struct JacobiPrec {
@@ -652,10 +687,14 @@ struct JacobiPrec {
 
 // Restarted GMRES(m), right preconditioned (prec.apply / prec.apply_add), CGS2
 // Arnoldi with device dot products, Givens rotations on the host.
+// With Zs (m vectors) the method is flexible GMRES (Saad 1993): the
+// preconditioned directions z_j = M^-1 v_j are kept and x += Z y, so the
+// residual estimate stays exact when M^-1 is not applied identically each time
+// (round-off growth inside a multigrid cycle).
 template <int NB, typename TM, typename TV, typename Prec>
 static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, const TV* b, TV* x, TV* V, TV* z,
                       TV* w, const Prec& prec, double* part, double* dh, int m, double tol, int maxiter,
-                      int* iters_out, double* relres_out, hipStream_t st) {
+                      int* iters_out, double* relres_out, hipStream_t st, TV* Zs = nullptr) {
   long n = (long)n_rows * NB;
   long ldv = n;
   int vb = dot_blocks(n);
@@ -710,9 +749,10 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
     for (; j < m && total < maxiter; ++j, ++total) {
       TV* vj = V + (long)j * ldv;
       TV* vj1 = V + (long)(j + 1) * ldv;
-      int prc = prec.apply(vj, z, st);
+      TV* zj = Zs ? Zs + (long)j * ldv : z;
+      int prc = prec.apply(vj, zj, st);
       if (prc) return prc;
-      launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, z, w, nullptr, nullptr, nullptr, st);
+      launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, zj, w, nullptr, nullptr, nullptr, st);
       // CGS2
       for (int pass = 0; pass < 2; ++pass) {
         k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, st>>>(n, j + 1, V, ldv, w, nchunk, part);
@@ -757,10 +797,16 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
       y[i] = s / H[(size_t)i * m + i];
     }
     GM_HIP(hipMemcpyAsync(dh, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
-    k_combine<TV><<<vb, RED_T, 0, st>>>(n, j, V, ldv, dh, w);
-    GM_LAUNCH_CHECK();
-    int prc = prec.apply_add(w, x, z, st);
-    if (prc) return prc;
+    if (Zs) {
+      k_combine<TV><<<vb, RED_T, 0, st>>>(n, j, Zs, ldv, dh, w);
+      k_add_into<TV><<<vb, RED_T, 0, st>>>(n, w, x);
+      GM_LAUNCH_CHECK();
+    } else {
+      k_combine<TV><<<vb, RED_T, 0, st>>>(n, j, V, ldv, dh, w);
+      GM_LAUNCH_CHECK();
+      int prc = prec.apply_add(w, x, z, st);
+      if (prc) return prc;
+    }
     GM_HIP(hipStreamSynchronize(st));
     if (rel <= tol) break;
   }
@@ -1071,6 +1117,16 @@ __global__ void k_jac_update(int n_rows, const T* dinv, const T* b, const T* ax,
   }
 }
 
+// x = Minv b for the coarsest level's dense inverse (one row per thread)
+template <typename T>
+__global__ void k_dense_mv(int n, const T* Minv, const T* b, T* x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < n; ++j) s += (double)Minv[(long)i * n + j] * (double)b[j];
+  x[i] = (T)s;
+}
+
 template <typename T>
 __global__ void k_axpy_t(long n, const T* a, T* x) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -1106,6 +1162,12 @@ static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
   const T* vals = (const T*)lv.vals;
   T* r = (T*)lv.r;
   bool coarsest = (l == cy.n_levels - 1);
+  if (coarsest && lv.dense_inv) {
+    int nd = n * NB;
+    k_dense_mv<T><<<gm_blocks(nd, 128), 128, 0, st>>>(nd, (const T*)lv.dense_inv, b, x);
+    GM_LAUNCH_CHECK();
+    return 0;
+  }
   int sweeps = coarsest ? cy.coarse_sweeps : cy.nu_pre;
   k_jac_first<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, cy.omega, x);
   for (int k = 1; k < sweeps; ++k) {
@@ -1238,12 +1300,12 @@ struct MGPrec {
 }  // namespace mg
 
 extern "C" int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre,
-                           int nu_post, int coarse_sweeps, double omega, const void* b, void* x, void* V, void* z,
-                           void* w, double* part, double* dh, int m, double tol, int maxiter, int* iters,
+                           int nu_post, int coarse_sweeps, double omega, const void* b, void* x, void* V, void* Z,
+                           void* z, void* w, double* part, double* dh, int m, double tol, int maxiter, int* iters,
                            double* relres, hipStream_t s) {
   int rc = mg_check(dim, nb, n_levels, levels);
   if (rc) return rc;
-  GM_REQUIRE(iters && relres && m >= 1 && m <= 256 && b && x && V && z && w, "gm_gmres_mg: bad argument");
+  GM_REQUIRE(iters && relres && m >= 1 && m <= 256 && b && x && V && Z && z && w, "gm_gmres_mg: bad argument");
   GM_REQUIRE(nu_pre >= 1 && nu_post >= 0 && coarse_sweeps >= 1, "gm_gmres_mg: bad smoother counts");
   mg::Cycle cy{dim, n_levels, nu_pre, nu_post, coarse_sweeps, omega, levels};
   const gm_mg_level& l0 = levels[0];
@@ -1254,14 +1316,213 @@ extern "C" int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_
       mg::MGPrec<D, B, float> pc{cy, n};                                                                        \
       return gmres_impl<B, float, float>(l0.n_nodes, l0.row_ptr, l0.col_idx, (const float*)l0.vals,             \
                                          (const float*)b, (float*)x, (float*)V, (float*)z, (float*)w, pc, part, \
-                                         dh, m, tol, maxiter, iters, relres, s);                                \
+                                         dh, m, tol, maxiter, iters, relres, s, (float*)Z);                     \
     }                                                                                                           \
     mg::MGPrec<D, B, double> pc{cy, n};                                                                         \
     return gmres_impl<B, double, double>(l0.n_nodes, l0.row_ptr, l0.col_idx, (const double*)l0.vals,            \
                                          (const double*)b, (double*)x, (double*)V, (double*)z, (double*)w, pc,  \
-                                         part, dh, m, tol, maxiter, iters, relres, s);                          \
+                                         part, dh, m, tol, maxiter, iters, relres, s, (double*)Z);              \
   } while (0)
   GM_MG_DISPATCH(GM_GMG);
 #undef GM_GMG
   return -1;
+}
+
+// ================================================================== u-p block preconditioner
+// For the coupled (u-p) Newton systems K = [[A, G], [D, C]] (A: nbu x nbu
+// displacement blocks, one pressure dof per node), a block lower-triangular
+// right preconditioner:
+//     z_u = MG(A)^-1 v_u,     z_p = S^-1 (v_p - D z_u),
+// with the displacement block solved by a multigrid V-cycle (elasticity) and
+// the Schur complement S = C - D A^-1 G approximated by its diagonal with A
+// replaced by its block diagonal (per node: C_II - sum_J D_IJ A_JJ^-1 G_JI).
+namespace up {
+
+// u-block of every block (rows/cols 0..NBU-1) and the u-dof free mask
+template <int NB, typename T>
+__global__ void k_extract(int n_rows, int nnzb, const T* vals, const uint8_t* freem, T* vals_u, uint8_t* freem_u) {
+  constexpr int NBU = NB - 1;
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q < nnzb) {
+#pragma unroll
+    for (int r = 0; r < NBU; ++r)
+#pragma unroll
+      for (int c = 0; c < NBU; ++c) vals_u[q * NBU * NBU + r * NBU + c] = vals[q * NB * NB + r * NB + c];
+  }
+  if (q < n_rows)
+#pragma unroll
+    for (int c = 0; c < NBU; ++c) freem_u[q * NBU + c] = freem[q * NB + c];
+}
+
+// sinv[I] = 1 / (C_II - sum_J D_IJ Ainv_JJ G_JI); fixed / inactive pressure dofs (identity rows) give 1.
+template <int NB, typename T>
+__global__ void k_schur_diag(int n_rows, const int* rp, const int* ci, const T* vals, const T* dinv_u,
+                             const uint8_t* freem, T* sinv) {
+  constexpr int NBU = NB - 1;
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n_rows) return;
+  double s = 0.0, cii = 1.0;
+  for (int q = rp[I]; q < rp[I + 1]; ++q) {
+    int J = ci[q];
+    const T* b = vals + (long)q * NB * NB;
+    if (J == I) cii = (double)b[(NB - 1) * NB + (NB - 1)];
+    double d[NBU];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < NBU; ++c) {
+      d[c] = (double)b[(NB - 1) * NB + c];
+      any |= (d[c] != 0.0);
+    }
+    if (!any) continue;
+    // G_JI: block (J, I), column NB-1, rows 0..NBU-1 (pattern is structurally symmetric)
+    int lo = rp[J], hi = rp[J + 1] - 1, qt = -1;
+    while (lo <= hi) {
+      int mid = (lo + hi) >> 1;
+      int cc = ci[mid];
+      if (cc == I) {
+        qt = mid;
+        break;
+      }
+      if (cc < I) lo = mid + 1;
+      else hi = mid - 1;
+    }
+    if (qt < 0) continue;
+    const T* bt = vals + (long)qt * NB * NB;
+    const T* ai = dinv_u + (long)J * NBU * NBU;
+#pragma unroll
+    for (int r = 0; r < NBU; ++r) {
+      double t = 0.0;
+#pragma unroll
+      for (int c = 0; c < NBU; ++c) t += (double)ai[r * NBU + c] * (double)bt[c * NB + (NB - 1)];
+      s += d[r] * t;
+    }
+  }
+  double S = freem[(long)I * NB + NB - 1] ? cii - s : 1.0;
+  sinv[I] = (T)((S != 0.0) ? 1.0 / S : 1.0);
+}
+
+template <int NB, typename T>
+__global__ void k_split(int n_rows, const T* v, T* vu) {
+  constexpr int NBU = NB - 1;
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n_rows) return;
+#pragma unroll
+  for (int c = 0; c < NBU; ++c) vu[(long)I * NBU + c] = v[(long)I * NB + c];
+}
+
+// z_p = sinv (v_p - D z_u); z = (z_u, z_p) interleaved (z_u may alias nothing in z)
+template <int NB, typename T>
+__global__ void k_pressure(int n_rows, const int* rp, const int* ci, const T* vals, const T* sinv, const T* v,
+                           const T* zu, T* z) {
+  constexpr int NBU = NB - 1;
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n_rows) return;
+  double s = 0.0;
+  for (int q = rp[I]; q < rp[I + 1]; ++q) {
+    int J = ci[q];
+    const T* b = vals + (long)q * NB * NB + (NB - 1) * NB;
+#pragma unroll
+    for (int c = 0; c < NBU; ++c) s += (double)b[c] * (double)zu[(long)J * NBU + c];
+  }
+#pragma unroll
+  for (int c = 0; c < NBU; ++c) z[(long)I * NB + c] = zu[(long)I * NBU + c];
+  z[(long)I * NB + NB - 1] = (T)((double)sinv[I] * ((double)v[(long)I * NB + NB - 1] - s));
+}
+
+template <int DIM, int NB, typename T>
+struct BlockPrec {
+  mg::Cycle cy;  // multigrid on the displacement block (nb = NB - 1)
+  int n_rows;
+  const int* rp;
+  const int* ci;
+  const T* vals;
+  const T* sinv;
+  T* vu;  // n_rows * (NB-1)
+  T* zu;  // n_rows * (NB-1)
+  long n;
+  int apply(const T* in, T* out, hipStream_t st) const {
+    int rb = gm_blocks(n_rows, 256);
+    k_split<NB, T><<<rb, 256, 0, st>>>(n_rows, in, vu);
+    GM_LAUNCH_CHECK();
+    int rc = mg::vcycle<DIM, NB - 1, T>(cy, 0, vu, zu, st);
+    if (rc) return rc;
+    k_pressure<NB, T><<<rb, 256, 0, st>>>(n_rows, rp, ci, vals, sinv, in, zu, out);
+    GM_LAUNCH_CHECK();
+    return 0;
+  }
+  int apply_add(const T* in, T* x, T* tmp, hipStream_t st) const {
+    int rc = apply(in, tmp, st);
+    if (rc) return rc;
+    mg::k_axpy_t<T><<<dot_blocks(n), RED_T, 0, st>>>(n, tmp, x);
+    GM_LAUNCH_CHECK();
+    return 0;
+  }
+};
+
+}  // namespace up
+
+// Extract the displacement block into u-level 0 (levels[0].vals / freem) of a
+// multigrid hierarchy with nb - 1 dofs per node, build that hierarchy, and the
+// inverse diagonal Schur approximation sinv (n_rows values).
+extern "C" int gm_up_setup(int dim, int nb, int fp32, int n_rows, int nnzb, const int* row_ptr, const int* col_idx,
+                           const void* vals, const uint8_t* freem, int n_levels, gm_mg_level* ulevels, void* ap_work,
+                           void* sinv, hipStream_t s) {
+  GM_REQUIRE(nb == 3 || nb == 4, "gm_up_setup: nb must be 3 (2D u-p) or 4 (3D u-p)");
+  GM_REQUIRE(dim == nb - 1, "gm_up_setup: nb must be dim + 1");
+  GM_REQUIRE(row_ptr && col_idx && vals && freem && sinv && ulevels, "gm_up_setup: null argument");
+  int rc = mg_check(dim, nb - 1, n_levels, ulevels);
+  if (rc) return rc;
+  GM_REQUIRE(ulevels[0].n_nodes == n_rows && ulevels[0].row_ptr == row_ptr && ulevels[0].col_idx == col_idx,
+             "gm_up_setup: u-level 0 must share the system's pattern");
+  long nt = nnzb > n_rows ? nnzb : n_rows;
+#define GM_UPS(NB_, T)                                                                                              \
+  do {                                                                                                              \
+    up::k_extract<NB_, T><<<gm_blocks(nt, 256), 256, 0, s>>>(n_rows, nnzb, (const T*)vals, freem,                   \
+                                                             (T*)ulevels[0].vals, ulevels[0].freem);               \
+    GM_LAUNCH_CHECK();                                                                                              \
+    rc = mg::setup<NB_ - 1, NB_ - 1, T>(n_levels, ulevels, ap_work, s);                                             \
+    if (rc) return rc;                                                                                              \
+    up::k_schur_diag<NB_, T><<<gm_blocks(n_rows, 256), 256, 0, s>>>(n_rows, row_ptr, col_idx, (const T*)vals,       \
+                                                                    (const T*)ulevels[0].dinv, freem, (T*)sinv);   \
+    GM_LAUNCH_CHECK();                                                                                              \
+    return 0;                                                                                                       \
+  } while (0)
+  if (nb == 3) {
+    if (fp32) GM_UPS(3, float);
+    GM_UPS(3, double);
+  }
+  if (fp32) GM_UPS(4, float);
+  GM_UPS(4, double);
+#undef GM_UPS
+}
+
+// Flexible GMRES(m) for a u-p system with the block preconditioner above.
+// vu, zu: n_rows * (nb-1) work vectors; other arguments as gm_gmres_mg.
+extern "C" int gm_gmres_up(int dim, int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx,
+                           const void* vals, const void* sinv, int n_levels, const gm_mg_level* ulevels, int nu_pre,
+                           int nu_post, int coarse_sweeps, double omega, void* vu, void* zu, const void* b, void* x,
+                           void* V, void* Z, void* z, void* w, double* part, double* dh, int m, double tol,
+                           int maxiter, int* iters, double* relres, hipStream_t s) {
+  GM_REQUIRE(nb == 3 || nb == 4, "gm_gmres_up: nb must be 3 or 4");
+  GM_REQUIRE(dim == nb - 1, "gm_gmres_up: nb must be dim + 1");
+  int rc = mg_check(dim, nb - 1, n_levels, ulevels);
+  if (rc) return rc;
+  GM_REQUIRE(iters && relres && m >= 1 && m <= 256 && b && x && V && Z && z && w && vu && zu && sinv,
+             "gm_gmres_up: bad argument");
+  GM_REQUIRE(nu_pre >= 1 && nu_post >= 0 && coarse_sweeps >= 1, "gm_gmres_up: bad smoother counts");
+  mg::Cycle cy{dim, n_levels, nu_pre, nu_post, coarse_sweeps, omega, ulevels};
+  long n = (long)n_rows * nb;
+#define GM_GUP(D, NB_, T)                                                                                      \
+  do {                                                                                                         \
+    up::BlockPrec<D, NB_, T> pc{cy, n_rows, row_ptr, col_idx, (const T*)vals, (const T*)sinv, (T*)vu, (T*)zu, n}; \
+    return gmres_impl<NB_, T, T>(n_rows, row_ptr, col_idx, (const T*)vals, (const T*)b, (T*)x, (T*)V, (T*)z,    \
+                                 (T*)w, pc, part, dh, m, tol, maxiter, iters, relres, s, (T*)Z);               \
+  } while (0)
+  if (nb == 3) {
+    if (fp32) GM_GUP(2, 3, float);
+    GM_GUP(2, 3, double);
+  }
+  if (fp32) GM_GUP(3, 4, float);
+  GM_GUP(3, 4, double);
+#undef GM_GUP
 }

@@ -205,7 +205,7 @@ def make_indent_bc():
 
 def indent3d(n_cells=(64, 64, 32), dx=1.0 / 64, total_steps=40, depth=0.05, footing=0.2, material='Drucker-Prager',
              E=2.0e4, precision='fp64', tol=1e-8, n_iter=10, mobility=1e-6, dt=1.0, gravity=0.0,
-             dp_apex='reference', shape_factor=0.0, clip_top=True):
+             dp_apex='reference', shape_factor=0.0, clip_top=True, krylov=None):
     ncx, ncy, ncz = n_cells
     nx, ny, nz = ncx + 4, ncy + 4, ncz + 3
     sx, sy, sz = dx, dx, 0.0
@@ -225,7 +225,8 @@ def indent3d(n_cells=(64, 64, 32), dx=1.0 / 64, total_steps=40, depth=0.05, foot
         solver='quasi_static_coupled_solver_3d',
         background_grid_dict=g,
         particles_dict=dict(n_particles=npart, PPD=PPD, p_vol=(dx / PPD) ** 3, p_rho=2.0),
-        iteration_dict=dict(n_iter=n_iter, tol=tol, tol_mode='relative', dt=dt, precision=precision),
+        iteration_dict=dict(n_iter=n_iter, tol=tol, tol_mode='relative', dt=dt, precision=precision,
+                            **({'krylov': krylov} if krylov else {})),
         material_dict=dict(youngs_modulus=E, poisson_ratio=0.3, material_name=material, friction_angle=30.0,
                            dilation_angle=0.0, cohesion=5.0, dp_apex=dp_apex, shape_factor=shape_factor),
         porous_media_dict=dict(phi_initial=0.5, mobility_constant=mobility),

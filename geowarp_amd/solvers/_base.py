@@ -13,7 +13,9 @@ Extensions over the reference (all optional, defaults reproduce it):
                                   FP32 Krylov, FP64 residual/refinement)
   iteration_dict['linear_tol'], ['linear_maxiter'], ['ir_steps'], ['forcing']
                                   ('fixed' | 'ew': Eisenstat-Walker inner tolerance),
-  ['krylov'] ('gmres' default | 'bicgstab'), ['krylov_restart']
+  ['krylov'] ('mg' default: multigrid-preconditioned flexible GMRES |
+  'gmres': block-Jacobi GMRES | 'bicgstab'), ['krylov_restart'],
+  ['mg_nu'], ['mg_omega'], ['mg_coarse_sweeps'], ['mg_ublock']  V-cycle options
   iteration_dict['verbose']       print residual norms like the reference
   material_dict['H']              J2 linear isotropic hardening modulus
   material_dict['friction_angle'], ['dilation_angle'], ['cohesion'],
@@ -92,8 +94,14 @@ class _QuasiStaticBase:
         self.linear_tol = it.get('linear_tol', 1e-10 if self.precision == 'fp64' else 1e-6)
         self.linear_maxiter = it.get('linear_maxiter', 20000)
         self.ir_steps = it.get('ir_steps', 1)
-        self.krylov_method = it.get('krylov', 'gmres')
-        self.krylov_restart = it.get('krylov_restart', 40)
+        # linear solver: flexible GMRES(krylov_restart) preconditioned by geometric-multigrid V-cycles
+        # ('mg', default), block-Jacobi GMRES ('gmres') or block-Jacobi BiCGStab ('bicgstab')
+        self.krylov_method = it.get('krylov', 'mg')
+        # multigrid V-cycle options (krylov='mg'): smoothing sweeps, damping, coarsest-level sweeps
+        mg_opts = {k: it[k2] for k, k2 in (('nu_pre', 'mg_nu'), ('nu_post', 'mg_nu'), ('omega', 'mg_omega'),
+                                           ('coarse_sweeps', 'mg_coarse_sweeps'), ('ublock', 'mg_ublock'))
+                   if k2 in it}
+        self.krylov_restart = it.get('krylov_restart', 250 if self.krylov_method == 'mg' else 40)
         # inner tolerance: 'fixed' (linear_tol) or 'ew' (Eisenstat-Walker choice 2)
         self.forcing = it.get('forcing', 'fixed' if self.precision == 'fp64' else 'ew')
         self.verbose = it.get('verbose', False)
@@ -116,6 +124,7 @@ class _QuasiStaticBase:
         spacing = (self.dx, self.dx, 0.0) if self.dim == 2 else (self.dx, self.dy, self.dz)
         n_grid = (self.n_grid_x, self.n_grid_y) if self.dim == 2 else (self.n_grid_x, self.n_grid_y, self.n_grid_z)
         self.engine = Engine(self.dim, self.coupled, n_grid, spacing, device=self.device, precision=self.precision)
+        self.engine.mg_options = mg_opts
         dev = self.device
         self.dofStruct = DofStruct(self.n_matrix_size, dev)
         if x_particles is None:
@@ -238,7 +247,8 @@ class _QuasiStaticBase:
             nr_prev = nr
             self.residual_history.append(nr)
             if self.verbose:
-                print('residual.norm:', nr)
+                print('residual.norm:', nr, 'krylov iterations:', eng.stats['krylov_iters'][-1],
+                      'relres: %.2e' % eng.stats['krylov_relres'][-1], flush=True)
             if not math.isfinite(nr):
                 raise L.GMError('Newton residual is not finite at iteration %d' % it)
             if r0 is None:

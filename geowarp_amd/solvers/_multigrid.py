@@ -36,11 +36,17 @@ def window_pattern(dim, cells):
 
 
 class Multigrid:
-    def __init__(self, eng, fp32, max_levels=8, min_cells=2, nu_pre=2, nu_post=2, coarse_sweeps=20, omega=0.7):
+    """Hierarchy for the whole system (single phase) or, with ``ublock=True``,
+    for the displacement block of a u-p system (nb - 1 dofs per node) inside the
+    block-triangular u-p preconditioner (gm_up_setup / gm_gmres_up)."""
+
+    def __init__(self, eng, fp32, max_levels=8, min_cells=2, nu_pre=3, nu_post=3, coarse_sweeps=20, omega=0.4,
+                 max_dense=3000, ublock=False):
         self.eng = eng
         self.fp32 = bool(fp32)
         self.dim = eng.dim
-        self.nb = eng.nb
+        self.ublock = bool(ublock) and eng.coupled
+        self.nb = eng.nb - 1 if self.ublock else eng.nb
         self.nu_pre, self.nu_post, self.coarse_sweeps, self.omega = nu_pre, nu_post, coarse_sweeps, omega
         g = eng.grid
         dims = [(g.nx, g.ny, g.nz if self.dim == 3 else 0)]
@@ -79,6 +85,20 @@ class Multigrid:
             self._keep.append(bufs)
         n0 = self.levels[0].n_nodes
         self.ap = torch.zeros(n0 * (4 ** self.dim) * nb2, dtype=dt, device=dev) if len(dims) > 1 else None
+        self.dt = dt
+        if self.ublock:
+            # level 0 holds the extracted displacement block; the Schur diagonal and split vectors
+            self._keep[0].update(vals=torch.zeros(0, dtype=dt, device=dev),
+                                 freem=torch.zeros(n0 * self.nb, dtype=torch.uint8, device=dev))
+            self.levels[0].freem = self._keep[0]['freem'].data_ptr()
+            self.sinv = torch.zeros(n0, dtype=dt, device=dev)
+            self.vu = torch.zeros(n0 * self.nb, dtype=dt, device=dev)
+            self.zu = torch.zeros(n0 * self.nb, dtype=dt, device=dev)
+        # the coarsest level is solved exactly through a dense inverse when it is small enough
+        nc = self.levels[len(dims) - 1].n_nodes * self.nb
+        self.dense = len(dims) > 1 and nc <= max_dense
+        self.cinv = torch.zeros(nc * nc, dtype=dt, device=dev) if self.dense else None
+        self.levels[len(dims) - 1].dense_inv = self.cinv.data_ptr() if self.dense else None
 
     @property
     def n_levels(self):
@@ -91,15 +111,53 @@ class Multigrid:
         lv.nnzb = eng.nnzb
         lv.row_ptr, lv.col_idx, lv.row_mask = eng.row_ptr.data_ptr(), eng.col_idx.data_ptr(), \
             eng.row_mask.data_ptr()
-        lv.vals, lv.freem = eng.vals.data_ptr(), eng.freem.data_ptr()
-        L.check(L.lib().gm_mg_setup(self.dim, self.nb, int(self.fp32), self.n_levels, self.levels,
-                                    L.ptr(self.ap), stream), 'gm_mg_setup')
+        if self.ublock:
+            k0 = self._keep[0]
+            need = max(eng.nnzb, 1) * self.nb * self.nb
+            if k0['vals'].numel() < need:
+                k0['vals'] = torch.zeros(int(need * 1.15) + 16, dtype=self.dt, device=eng.dev)
+            lv.vals = k0['vals'].data_ptr()
+            L.check(L.lib().gm_up_setup(self.dim, eng.nb, int(self.fp32), eng.n_nodes, eng.nnzb,
+                                        L.ptr(eng.row_ptr), L.ptr(eng.col_idx), L.ptr(eng.vals), L.ptr(eng.freem),
+                                        self.n_levels, self.levels, L.ptr(self.ap), L.ptr(self.sinv), stream),
+                    'gm_up_setup')
+        else:
+            lv.vals, lv.freem = eng.vals.data_ptr(), eng.freem.data_ptr()
+            L.check(L.lib().gm_mg_setup(self.dim, self.nb, int(self.fp32), self.n_levels, self.levels,
+                                        L.ptr(self.ap), stream), 'gm_mg_setup')
+        if self.dense:
+            # exact coarsest solve: pseudo-inverse of the (small) coarsest operator, formed on the host
+            k = self._keep[-1]
+            lv = self.levels[self.n_levels - 1]
+            nb, n = self.nb, lv.n_nodes
+            rp = k['rp'].cpu().numpy().astype(np.int64)
+            ci = k['ci'].cpu().numpy().astype(np.int64)
+            v = k['vals'].double().cpu().numpy().reshape(-1, nb, nb)
+            A = np.zeros((n * nb, n * nb))
+            brow = np.repeat(np.arange(n), np.diff(rp))
+            for r in range(nb):
+                for c in range(nb):
+                    A[brow * nb + r, ci * nb + c] += v[:, r, c]
+            inv = np.linalg.pinv(A)
+            self.cinv.copy_(torch.from_numpy(inv.reshape(-1)).to(self.cinv.dtype))
 
     def gmres(self, b, x, V, z, w, part, dh, m, tol, maxiter, stream):
+        eng = self.eng
+        n = self.levels[0].n_nodes * eng.nb
+        if getattr(self, '_Z', None) is None or self._Z.numel() < m * n:
+            self._Z = torch.zeros(m * n, dtype=V.dtype, device=V.device)
         it = L.c_int(0)
         rr = L.c_double(0.0)
+        if self.ublock:
+            L.check(L.lib().gm_gmres_up(self.dim, eng.nb, int(self.fp32), eng.n_nodes, L.ptr(eng.row_ptr),
+                                        L.ptr(eng.col_idx), L.ptr(eng.vals), L.ptr(self.sinv), self.n_levels,
+                                        self.levels, self.nu_pre, self.nu_post, self.coarse_sweeps, self.omega,
+                                        L.ptr(self.vu), L.ptr(self.zu), L.ptr(b), L.ptr(x), L.ptr(V),
+                                        L.ptr(self._Z), L.ptr(z), L.ptr(w), L.ptr(part), L.ptr(dh), m, tol,
+                                        maxiter, L.ctypes.byref(it), L.ctypes.byref(rr), stream), 'gm_gmres_up')
+            return it.value, rr.value
         L.check(L.lib().gm_gmres_mg(self.dim, self.nb, int(self.fp32), self.n_levels, self.levels, self.nu_pre,
                                     self.nu_post, self.coarse_sweeps, self.omega, L.ptr(b), L.ptr(x), L.ptr(V),
-                                    L.ptr(z), L.ptr(w), L.ptr(part), L.ptr(dh), m, tol, maxiter,
+                                    L.ptr(self._Z), L.ptr(z), L.ptr(w), L.ptr(part), L.ptr(dh), m, tol, maxiter,
                                     L.ctypes.byref(it), L.ctypes.byref(rr), stream), 'gm_gmres_mg')
         return it.value, rr.value

@@ -233,12 +233,30 @@ int gm_mg_setup(int dim, int nb, int fp32, int n_levels, gm_mg_level* levels, vo
 int gm_mg_vcycle(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre, int nu_post,
                  int coarse_sweeps, double omega, const void* b, void* x, hipStream_t stream);
 
-/* GMRES(m) on level 0's matrix, right-preconditioned by the V-cycle; arguments
-   as gm_gmres (V: (m+1) * n vectors, z, w: n; part >= 64(m+1) and >= 2048
-   doubles; dh >= 3(m+2) doubles). */
+/* Flexible GMRES(m) on level 0's matrix, right-preconditioned by the V-cycle;
+   arguments as gm_gmres (V: (m+1) * n vectors, Z: m * n preconditioned
+   directions, z, w: n; part >= 64(m+1) and >= 2048 doubles; dh >= 3(m+2)
+   doubles). */
 int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre, int nu_post,
-                int coarse_sweeps, double omega, const void* b, void* x, void* V, void* z, void* w, double* part,
-                double* dh, int m, double tol, int maxiter, int* iters, double* relres, hipStream_t stream);
+                int coarse_sweeps, double omega, const void* b, void* x, void* V, void* Z, void* z, void* w,
+                double* part, double* dh, int m, double tol, int maxiter, int* iters, double* relres,
+                hipStream_t stream);
+
+/* u-p systems (nb = dim + 1): block lower-triangular preconditioner
+   z_u = MG(A_uu)^-1 v_u, z_p = S^-1 (v_p - K_pu z_u), S the diagonal Schur
+   approximation C - D blockdiag(A)^-1 G.  gm_up_setup extracts the displacement
+   block into ulevels[0] (which shares the system's row_ptr/col_idx/row_mask and
+   owns vals/freem with nb-1 dofs per node), builds the hierarchy and writes
+   sinv (n_rows); gm_gmres_up runs flexible GMRES(m) with it (vu, zu: n_rows *
+   (nb-1) work vectors; other arguments as gm_gmres_mg). */
+int gm_up_setup(int dim, int nb, int fp32, int n_rows, int nnzb, const int* row_ptr, const int* col_idx,
+                const void* vals, const uint8_t* freem, int n_levels, gm_mg_level* ulevels, void* ap_work,
+                void* sinv, hipStream_t stream);
+int gm_gmres_up(int dim, int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
+                const void* sinv, int n_levels, const gm_mg_level* ulevels, int nu_pre, int nu_post,
+                int coarse_sweeps, double omega, void* vu, void* zu, const void* b, void* x, void* V, void* Z,
+                void* z, void* w, double* part, double* dh, int m, double tol, int maxiter, int* iters,
+                double* relres, hipStream_t stream);
 
 #ifdef __cplusplus
 }
