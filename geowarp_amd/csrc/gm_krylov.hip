@@ -1526,3 +1526,57 @@ extern "C" int gm_gmres_up(int dim, int nb, int fp32, int n_rows, const int* row
   GM_GUP(3, 4, double);
 #undef GM_GUP
 }
+
+// ================================================================== row equilibration
+// Scale every row of K (and of the right-hand side) by 1/|K_ii|, so that
+// displacement and pressure equations, whose magnitudes differ by many orders,
+// weigh alike in the Krylov residual norm; the solution is unchanged.
+template <int NB, typename TM>
+__global__ void k_row_scale(int n_rows, const int* rp, const int* ci, TM* vals, const double* rhs_in,
+                            double* rhs_out) {
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n_rows) return;
+  int qd = -1;
+  for (int q = rp[I]; q < rp[I + 1]; ++q)
+    if (ci[q] == I) {
+      qd = q;
+      break;
+    }
+  double sc[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    double d = (qd >= 0) ? fabs((double)vals[(long)qd * NB * NB + r * NB + r]) : 0.0;
+    sc[r] = (d > 0.0) ? 1.0 / d : 1.0;
+  }
+  for (int q = rp[I]; q < rp[I + 1]; ++q)
+#pragma unroll
+    for (int r = 0; r < NB; ++r)
+#pragma unroll
+      for (int c = 0; c < NB; ++c) vals[(long)q * NB * NB + r * NB + c] = (TM)((double)vals[(long)q * NB * NB + r * NB + c] * sc[r]);
+#pragma unroll
+  for (int r = 0; r < NB; ++r) rhs_out[(long)I * NB + r] = rhs_in[(long)I * NB + r] * sc[r];
+}
+
+extern "C" int gm_row_scale(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, void* vals,
+                            const double* rhs_in, double* rhs_out, hipStream_t s) {
+  GM_REQUIRE(row_ptr && col_idx && vals && rhs_in && rhs_out, "gm_row_scale: null argument");
+  if (n_rows == 0) return 0;
+#define GM_RS(NB)                                                                                               \
+  if (nb == NB) {                                                                                               \
+    if (fp32)                                                                                                   \
+      k_row_scale<NB, float><<<gm_blocks(n_rows, 256), 256, 0, s>>>(n_rows, row_ptr, col_idx, (float*)vals,     \
+                                                                     rhs_in, rhs_out);                          \
+    else                                                                                                        \
+      k_row_scale<NB, double><<<gm_blocks(n_rows, 256), 256, 0, s>>>(n_rows, row_ptr, col_idx, (double*)vals,   \
+                                                                      rhs_in, rhs_out);                         \
+    GM_LAUNCH_CHECK();                                                                                          \
+    return 0;                                                                                                   \
+  }
+  GM_RS(1)
+  GM_RS(2)
+  GM_RS(3)
+  GM_RS(4)
+#undef GM_RS
+  gm::set_error("gm_row_scale: unsupported block size %d", nb);
+  return -1;
+}

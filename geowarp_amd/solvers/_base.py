@@ -97,6 +97,11 @@ class _QuasiStaticBase:
         # linear solver: flexible GMRES(krylov_restart) preconditioned by geometric-multigrid V-cycles
         # ('mg', default), block-Jacobi GMRES ('gmres') or block-Jacobi BiCGStab ('bicgstab')
         self.krylov_method = it.get('krylov', 'mg')
+        # row equilibration of the Newton system (u-p: displacement and pressure rows differ by orders)
+        self.equilibrate = it.get('equilibrate', False)
+        # backtracking line search on the Newton correction (the reference takes full steps)
+        self.line_search = it.get('line_search', False)
+        self.line_search_steps = []
         # multigrid V-cycle options (krylov='mg'): smoothing sweeps, damping, coarsest-level sweeps
         mg_opts = {k: it[k2] for k, k2 in (('nu_pre', 'mg_nu'), ('nu_post', 'mg_nu'), ('omega', 'mg_omega'),
                                            ('coarse_sweeps', 'mg_coarse_sweeps'), ('ublock', 'mg_ublock'))
@@ -240,10 +245,16 @@ class _QuasiStaticBase:
                 eta = self._ew_forcing(nr, nr_prev, eta, r0 if r0 is not None else nr)
                 tol_lin = eta
             eng.solve(tol_lin, self.linear_maxiter, ir_steps=self.ir_steps, method=self.krylov_method,
-                      restart=self.krylov_restart)
-            eng.axpy(1.0, eng.delta, u_new)
+                      restart=self.krylov_restart, equilibrate=self.equilibrate)
             if tm is not None:
                 tm.mark('krylov')
+            converged_here = (nr if self.tol_mode == 'absolute' else (nr / r0 if r0 else 1.0)) < self.tol
+            if self.line_search and not converged_here and it + 1 < self.n_iter:
+                self._line_search(args, u_new, nr, after_residual)
+            else:
+                eng.axpy(1.0, eng.delta, u_new)
+            if tm is not None:
+                tm.mark('line_search')
             nr_prev = nr
             self.residual_history.append(nr)
             if self.verbose:
@@ -257,6 +268,25 @@ class _QuasiStaticBase:
             if crit < self.tol:
                 break
         self.newton_iterations = it + 1
+
+    def _line_search(self, args, u_new, nr, after_residual, max_halvings=6):
+        """Backtracking on the Newton correction (not in the reference, which
+        always takes the full step): accept u + a*delta for the first
+        a = 1, 1/2, ... whose residual norm is finite and below (1 - 1e-4 a) nr."""
+        eng = self.engine
+        base = u_new.clone()
+        a = 1.0
+        for _ in range(max_halvings + 1):
+            u_new.copy_(base)
+            eng.axpy(a, eng.delta, u_new)
+            eng.stage(args, want_tangent=False)
+            eng.residual(self.x_particles, self.GIMP_lp)
+            eng.norm2(eng.rhs, 0)
+            trial = math.sqrt(max(eng.scal[0].item(), 0.0))
+            if math.isfinite(trial) and trial <= (1.0 - 1e-4 * a) * nr:
+                break
+            a *= 0.5
+        self.line_search_steps.append(a)
 
     def _ew_forcing(self, nr, nr_prev, eta_prev, r0):
         """Eisenstat & Walker (1996) choice 2 forcing term with their safeguards,

@@ -303,14 +303,25 @@ class Engine:
                                     check_every, L.ctypes.byref(it), L.ctypes.byref(rr), _stream()), 'gm_bicgstab')
         return it.value, rr.value
 
-    def solve(self, tol, maxiter, check_every=8, ir_steps=1, method='gmres', restart=40):
+    def solve(self, tol, maxiter, check_every=8, ir_steps=1, method='gmres', restart=40, equilibrate=False):
         """Solve K delta = rhs (K = -dR/du).  FP64: Krylov in FP64.  Mixed:
         Krylov in FP32 on the FP32 Jacobian, wrapped in `ir_steps` sweeps of
-        iterative refinement whose residual rhs - K delta is formed in FP64."""
+        iterative refinement whose residual rhs - K delta is formed in FP64.
+        equilibrate: scale rows by 1/|K_ii| first (K is overwritten; the
+        residual vector is not)."""
         lib = L.lib()
         n = self.n_nodes * self.nb
+        b = self.rhs
+        if equilibrate:
+            if getattr(self, '_rhs_s', None) is None or self._rhs_s.numel() != n:
+                self._rhs_s = torch.zeros(n, dtype=torch.float64, device=self.dev)
+            L.check(lib.gm_row_scale(self.nb, int(self.precision == 'mixed'), self.n_nodes, L.ptr(self.row_ptr),
+                                     L.ptr(self.col_idx), L.ptr(self.vals), L.ptr(self.rhs), L.ptr(self._rhs_s),
+                                     _stream()), 'gm_row_scale')
+            self.jac_version += 1
+            b = self._rhs_s
         if self.precision == 'fp64':
-            its, rel = self.krylov(self.rhs, self.delta, False, tol, maxiter, method, restart, check_every)
+            its, rel = self.krylov(b, self.delta, False, tol, maxiter, method, restart, check_every)
             self.stats['krylov_iters'].append(its)
             self.stats['krylov_relres'].append(rel)
             return
@@ -323,10 +334,10 @@ class Engine:
         total = 0
         for k in range(ir_steps):
             if k == 0:
-                src = self.rhs
+                src = b
             else:
                 L.check(lib.gm_residual_refine(self.nb, 1, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
-                                               L.ptr(self.vals), L.ptr(self.rhs), L.ptr(self.delta), L.ptr(self._ax64),
+                                               L.ptr(self.vals), L.ptr(b), L.ptr(self.delta), L.ptr(self._ax64),
                                                L.ptr(self._res64), _stream()), 'gm_residual_refine')
                 src = self._res64
             L.check(lib.gm_d2f(n, L.ptr(src), L.ptr(self._b32), _stream()), 'gm_d2f')

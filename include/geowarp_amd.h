@@ -258,6 +258,13 @@ int gm_gmres_up(int dim, int nb, int fp32, int n_rows, const int* row_ptr, const
                 void* z, void* w, double* part, double* dh, int m, double tol, int maxiter, int* iters,
                 double* relres, hipStream_t stream);
 
+/* Row equilibration of the Newton system: K <- diag(1/|K_ii|) K in place and
+   rhs_out = diag(1/|K_ii|) rhs_in (the solution is unchanged; displacement
+   and pressure rows then weigh alike in the Krylov residual).  No reference
+   counterpart: the reference hands K to a direct or Jacobi-BiCGStab solver. */
+int gm_row_scale(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, void* vals,
+                 const double* rhs_in, double* rhs_out, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
