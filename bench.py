@@ -161,6 +161,17 @@ def adjoint_bench(steps, total=30, every=5, n_specimens=64, krylov='mg'):
                 grad_norm=dict(phi=float(phi_t.grad.norm()), coh=float(coh_t.grad.norm())))
 
 
+def measured_traffic(cells, material):
+    """HBM bytes per launch of the level-0 FP32 SpMV from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/k_spmv_c4_indentation*),
+    scaled by the calibration kernel; None for other workloads."""
+    path = os.path.join(ROOT, 'profiles', 'k_spmv_c4_indentation_traffic.json')
+    if tuple(cells) != (64, 64, 32) or material != 'Neo-Hookean' or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)['traffic_bytes_per_launch']
+
+
 def field_errors(a, b):
     import torch
     out = {}
@@ -261,7 +272,8 @@ def main():
         'mixed': res.get('mixed'),
         'fp64': res.get('fp64'),
         'roofline': {'bound': 'hbm', 'achieved': round(roof['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': round(roof['achieved_GBs'] / HBM_PEAK_GBS, 4), 'traffic': None,
+                     'unit': 'GB/s', 'frac': round(roof['achieved_GBs'] / HBM_PEAK_GBS, 4),
+                     'traffic': measured_traffic(cells, args.material),
                      'kernel': roof['kernel'], 'avg_us': round(roof['avg_us'], 2),
                      'algorithmic_bytes': roof['bytes_per_launch']},
     }
