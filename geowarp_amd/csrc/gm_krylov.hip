@@ -1177,15 +1177,18 @@ static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
   GM_LAUNCH_CHECK();
   if (coarsest) return 0;
   const gm_mg_level& cv = cy.L[l + 1];
-  launch_spmv<NB, T, T, T>(n, lv.row_ptr, lv.col_idx, vals, x, r, nullptr, nullptr, nullptr, st);
-  long nv = (long)n * NB;
-  k_sub_inplace<T><<<dot_blocks(nv), RED_T, 0, st>>>(nv, b, r);
-  k_restrict<DIM, NB, T><<<gm_blocks(cv.n_nodes, 256), 256, 0, st>>>(dims_of(lv), dims_of(cv), cv.n_nodes, lv.freem,
-                                                                      r, (T*)cv.b);
-  GM_LAUNCH_CHECK();
-  int rc = vcycle<DIM, NB, T>(cy, l + 1, (const T*)cv.b, (T*)cv.x, st);
-  if (rc) return rc;
-  k_prolong_add<DIM, NB, T><<<rb, 256, 0, st>>>(dims_of(lv), dims_of(cv), n, lv.freem, (const T*)cv.x, x);
+  const int gamma = lv.gamma > 1 ? lv.gamma : 1;  // 1: V-cycle, 2: W-cycle at this level
+  for (int g = 0; g < gamma; ++g) {
+    launch_spmv<NB, T, T, T>(n, lv.row_ptr, lv.col_idx, vals, x, r, nullptr, nullptr, nullptr, st);
+    long nv = (long)n * NB;
+    k_sub_inplace<T><<<dot_blocks(nv), RED_T, 0, st>>>(nv, b, r);
+    k_restrict<DIM, NB, T><<<gm_blocks(cv.n_nodes, 256), 256, 0, st>>>(dims_of(lv), dims_of(cv), cv.n_nodes,
+                                                                        lv.freem, r, (T*)cv.b);
+    GM_LAUNCH_CHECK();
+    int rc = vcycle<DIM, NB, T>(cy, l + 1, (const T*)cv.b, (T*)cv.x, st);
+    if (rc) return rc;
+    k_prolong_add<DIM, NB, T><<<rb, 256, 0, st>>>(dims_of(lv), dims_of(cv), n, lv.freem, (const T*)cv.x, x);
+  }
   for (int k = 0; k < cy.nu_post; ++k) {
     launch_spmv<NB, T, T, T>(n, lv.row_ptr, lv.col_idx, vals, x, r, nullptr, nullptr, nullptr, st);
     k_jac_update<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, r, cy.omega, x);

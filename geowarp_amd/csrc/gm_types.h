@@ -145,6 +145,7 @@ typedef struct gm_mg_level {
   uint8_t* freem;       /* n_nodes * nb: 1 = free dof */
   void* x; void* b; void* r;  /* n_nodes * nb work vectors (level 0 x/b unused) */
   const void* dense_inv;      /* coarsest level only, may be null: (n_nodes*nb)^2 inverse, row-major */
+  int gamma;                  /* coarse-grid corrections from this level per cycle (<= 1: V, 2: W) */
 } gm_mg_level;
 
 #ifdef __cplusplus

@@ -41,7 +41,7 @@ class Multigrid:
     block-triangular u-p preconditioner (gm_up_setup / gm_gmres_up)."""
 
     def __init__(self, eng, fp32, max_levels=8, min_cells=2, nu_pre=3, nu_post=3, coarse_sweeps=20, omega=0.4,
-                 max_dense=3000, ublock=False):
+                 max_dense=3000, ublock=False, gamma=1):
         self.eng = eng
         self.fp32 = bool(fp32)
         self.dim = eng.dim
@@ -66,6 +66,7 @@ class Multigrid:
             n = (nx + 1) * (ny + 1) * ((nz + 1) if self.dim == 3 else 1)
             lv = self.levels[l]
             lv.nx, lv.ny, lv.nz, lv.n_nodes = nx, ny, nz, n
+            lv.gamma = int(gamma)
             bufs = dict(dinv=torch.zeros(n * nb2, dtype=dt, device=dev),
                         r=torch.zeros(n * self.nb, dtype=dt, device=dev))
             if l > 0:
