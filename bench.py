@@ -50,6 +50,7 @@ def parse():
     ap.add_argument('--adjoint-steps', type=int, default=30,
                     help='load steps of the batched-specimen adjoint measurement (0 = skip)')
     ap.add_argument('--krylov', type=str, default='mg', help="'mg' (multigrid GMRES) or 'gmres' (block Jacobi)")
+    ap.add_argument('--c2-steps', type=int, default=2, help='load steps of the 2D J2 bar measurement (0 = skip)')
     return ap.parse_args()
 
 
@@ -159,6 +160,42 @@ def adjoint_bench(steps, total=30, every=5, n_specimens=64, krylov='mg'):
                 forward_steps=adj.stats['forward_steps'], adjoint_steps=adj.stats['adjoint_steps'],
                 transposed_solve_iters=adj.stats['krylov_iters'],
                 grad_norm=dict(phi=float(phi_t.grad.norm()), coh=float(coh_t.grad.norm())))
+
+
+def c2_bench(steps):
+    """2D plane-strain J2 bar (256 x 1024 cells of 1.953125 mm, 1,048,576 particles,
+    E 10 MPa, sigma_y 20 kPa, H 100 kPa, top platen 10 % of the height in 50
+    steps): load steps 0..steps-1 in the FP64 and the mixed form from the same
+    initial state; times, Newton iterations and field errors of the mixed form."""
+    import torch
+    import geowarp_amd.solvers as gs
+    from geowarp_amd import problems as P
+    out, solvers = {}, {}
+    for form in ('fp64', 'mixed'):
+        cfg = P.bar_j2(precision=form)
+        s = P.build_solver(gs, cfg, device='cuda:0')
+        cfg['set_boundary_dofs'](s.dofStruct.boundary_flag_array, s.n_grid_x, s.n_nodes)
+        s.line_search = True
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        iters = 0
+        for k in range(steps):
+            s.advance_one_step(k, cfg['total_steps'])
+            iters += s.newton_iterations
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[form] = dict(seconds=round(dt, 4), newton_iters=iters, newton_per_s=round(iters / dt, 4),
+                         last_residuals=['%.2e' % v for v in s.residual_history])
+        solvers[form] = s
+    a, b = solvers['mixed'], solvers['fp64']
+    du = float(torch.linalg.norm((a.x_particles - a.x0) - (b.x_particles - b.x0)) / torch.linalg.norm(b.x_particles - b.x0))
+    ds = float(torch.linalg.norm(a.particle_Cauchy_stress_array - b.particle_Cauchy_stress_array)
+               / torch.linalg.norm(b.particle_Cauchy_stress_array))
+    out['mixed_vs_fp64_speedup'] = round(out['fp64']['seconds'] / out['mixed']['seconds'], 3)
+    out['mixed_vs_fp64_field_rel_err'] = dict(displacement=du, stress=ds)
+    out['workload'] = ('2D J2 bar 256x1024 cells, %d particles, platen 10%% of the height in 50 steps, load steps '
+                       '0-%d to 1e-8 relative residual' % (b.n_particles, steps - 1))
+    return out
 
 
 def measured_traffic(cells, material):
@@ -284,6 +321,8 @@ def main():
     if 'fp64' in res:
         out['mixed_vs_fp64_speedup'] = round(res['fp64']['seconds'] / res['mixed']['seconds'], 3)
         out['mixed_vs_fp64_field_rel_err'] = field_errors(solvers['mixed'], solvers['fp64'])
+    if args.c2_steps > 0:
+        out['c2_j2_bar'] = c2_bench(args.c2_steps)
     if args.adjoint_steps > 0:
         out['adjoint'] = adjoint_bench(args.adjoint_steps, krylov=args.krylov)
     if not args.no_cpu:

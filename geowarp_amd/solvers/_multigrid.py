@@ -35,6 +35,21 @@ def window_pattern(dim, cells):
     return row_ptr, col_idx, m.view(np.int64)
 
 
+def _robust_inverse(A):
+    """Inverse of the small coarsest operator: LU when it is regular, else a
+    pseudo-inverse from the (slower, more robust) QR-iteration SVD driver."""
+    try:
+        inv = np.linalg.inv(A)
+        if np.all(np.isfinite(inv)):
+            return inv
+    except np.linalg.LinAlgError:
+        pass
+    import scipy.linalg
+    u, sv, vt = scipy.linalg.svd(A, full_matrices=False, lapack_driver='gesvd')
+    keep = sv > sv.max() * max(A.shape) * np.finfo(np.float64).eps
+    return (vt[keep].T / sv[keep]) @ u[:, keep].T
+
+
 class Multigrid:
     """Hierarchy for the whole system (single phase) or, with ``ublock=True``,
     for the displacement block of a u-p system (nb - 1 dofs per node) inside the
@@ -139,8 +154,7 @@ class Multigrid:
             for r in range(nb):
                 for c in range(nb):
                     A[brow * nb + r, ci * nb + c] += v[:, r, c]
-            inv = np.linalg.pinv(A)
-            self.cinv.copy_(torch.from_numpy(inv.reshape(-1)).to(self.cinv.dtype))
+            self.cinv.copy_(torch.from_numpy(_robust_inverse(A).reshape(-1)).to(self.cinv.dtype))
 
     def gmres(self, b, x, V, z, w, part, dh, m, tol, maxiter, stream):
         eng = self.eng
