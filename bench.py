@@ -184,8 +184,10 @@ def c2_bench(steps):
             iters += s.newton_iterations
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        hist = s.residual_history
+        conv = bool(hist and hist[0] > 0 and hist[-1] / hist[0] < s.tol)
         out[form] = dict(seconds=round(dt, 4), newton_iters=iters, newton_per_s=round(iters / dt, 4),
-                         last_residuals=['%.2e' % v for v in s.residual_history])
+                         converged=conv, last_residuals=['%.2e' % v for v in hist])
         solvers[form] = s
     a, b = solvers['mixed'], solvers['fp64']
     du = float(torch.linalg.norm((a.x_particles - a.x0) - (b.x_particles - b.x0)) / torch.linalg.norm(b.x_particles - b.x0))

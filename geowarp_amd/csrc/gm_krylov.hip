@@ -767,6 +767,13 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
       GM_HIP(hipStreamSynchronize(st));
       for (int i = 0; i <= j; ++i) hcol[i] = hh[i] + hh[(m + 2) + i];
       double hn = sqrt(fmax(hh[2 * (m + 2)], 0.0));
+      if (!(hn == hn) || hn > 1e300 || !(hcol[j] == hcol[j])) {
+        gm::set_error("gmres: Arnoldi vector became non-finite after %d iterations (preconditioner overflow)",
+                      total);
+        *iters_out = total;
+        *relres_out = hn;
+        return -6;
+      }
       hcol[j + 1] = hn;
       if (hn > 0.0) k_scale_copy<TV><<<vb, RED_T, 0, st>>>(n, w, 1.0 / hn, vj1);
       // apply previous rotations, then a new one

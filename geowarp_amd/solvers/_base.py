@@ -93,7 +93,9 @@ class _QuasiStaticBase:
         self.precision = it.get('precision', 'fp64')
         self.linear_tol = it.get('linear_tol', 1e-10 if self.precision == 'fp64' else 1e-6)
         self.linear_maxiter = it.get('linear_maxiter', 20000)
-        self.ir_steps = it.get('ir_steps', 1)
+        # FP64 iterative-refinement sweeps around the FP32 Krylov solve (mixed form); a sweep
+        # ends the loop once the FP64 residual of the correction meets the inner tolerance
+        self.ir_steps = it.get('ir_steps', 1 if self.precision == 'fp64' else 4)
         # linear solver: flexible GMRES(krylov_restart) preconditioned by geometric-multigrid V-cycles
         # ('mg', default), block-Jacobi GMRES ('gmres') or block-Jacobi BiCGStab ('bicgstab')
         self.krylov_method = it.get('krylov', 'mg')
@@ -109,6 +111,7 @@ class _QuasiStaticBase:
         self.krylov_restart = it.get('krylov_restart', 250 if self.krylov_method == 'mg' else 40)
         # inner tolerance: 'fixed' (linear_tol) or 'ew' (Eisenstat-Walker choice 2)
         self.forcing = it.get('forcing', 'fixed' if self.precision == 'fp64' else 'ew')
+        self.eta_max = it.get('forcing_max', 1e-2)  # cap of the Eisenstat-Walker inner tolerance
         self.verbose = it.get('verbose', False)
         m = material_dict
         self.material_name = m['material_name']
@@ -290,8 +293,8 @@ class _QuasiStaticBase:
 
     def _ew_forcing(self, nr, nr_prev, eta_prev, r0):
         """Eisenstat & Walker (1996) choice 2 forcing term with their safeguards,
-        clamped to [linear_tol, 0.1], and not tighter than needed to reach the
-        Newton tolerance."""
+        clamped to [linear_tol, forcing_max], and not tighter than needed to
+        reach the Newton tolerance."""
         gamma, alpha = 0.9, 2.0
         if nr_prev is None or eta_prev is None:
             eta = 1e-2
@@ -303,7 +306,7 @@ class _QuasiStaticBase:
         target = self.tol * r0 if self.tol_mode == 'relative' else self.tol
         if nr > 0:
             eta = max(eta, 0.5 * target / nr)
-        return min(max(eta, self.linear_tol), 0.1)
+        return min(max(eta, self.linear_tol), self.eta_max)
 
     # ------------------------------------------------------------------ dof-major views
     def _sync_flags_out(self):

@@ -92,6 +92,7 @@ class Engine:
         self.stats = dict(krylov_iters=[], krylov_relres=[])
         self.jac_version = 0
         self.mg_options = {}
+        self.mixed_inner_maxiter = 400  # FP32 Krylov iterations per refinement sweep
 
     # ------------------------------------------------------------------ buffers
     def _alloc_vals(self):
@@ -276,6 +277,23 @@ class Engine:
         return self._ws[key]
 
     def krylov(self, b, x, fp32, tol, maxiter, method='gmres', restart=40, check_every=8):
+        """Krylov solve with a fallback: when the multigrid-preconditioned
+        solve fails (non-finite Arnoldi vectors or a residual that grows) the
+        system is solved again with block-Jacobi GMRES."""
+        try:
+            its, rel = self._krylov(b, x, fp32, tol, maxiter, method, restart, check_every)
+        except L.GMError:
+            if method != 'mg':
+                raise
+            its, rel = maxiter, float('inf')
+        if method == 'mg' and not (rel <= 1.0):
+            self.stats.setdefault('fallbacks', 0)
+            self.stats['fallbacks'] += 1
+            its2, rel = self._krylov(b, x, fp32, tol, maxiter, 'gmres', min(restart, 100), check_every)
+            its += its2
+        return its, rel
+
+    def _krylov(self, b, x, fp32, tol, maxiter, method='gmres', restart=40, check_every=8):
         """One Krylov solve K x = b on the current Jacobian: GMRES(restart)
         preconditioned by geometric-multigrid V-cycles ('mg') or block Jacobi
         ('gmres'), or block-Jacobi BiCGStab ('bicgstab'); returns (iterations, relres)."""
@@ -332,17 +350,29 @@ class Engine:
             self._ax64 = torch.zeros(n, dtype=torch.float64, device=self.dev)
         self.delta.zero_()
         total = 0
+        bnorm = None
+        inner_max = min(maxiter, self.mixed_inner_maxiter)
+        rel_in = float('inf')
         for k in range(ir_steps):
             if k == 0:
                 src = b
             else:
+                if rel_in <= tol:
+                    break  # the FP32 solve met its tolerance: no further sweep
+                # FP64 refinement residual b - K delta; stop once it meets the tolerance
                 L.check(lib.gm_residual_refine(self.nb, 1, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
                                                L.ptr(self.vals), L.ptr(b), L.ptr(self.delta), L.ptr(self._ax64),
                                                L.ptr(self._res64), _stream()), 'gm_residual_refine')
                 src = self._res64
+                if bnorm is None:
+                    bnorm = float(torch.linalg.norm(b))
+                rel = float(torch.linalg.norm(self._res64)) / bnorm if bnorm > 0 else 0.0
+                if rel <= tol:
+                    break
             L.check(lib.gm_d2f(n, L.ptr(src), L.ptr(self._b32), _stream()), 'gm_d2f')
-            its, rel = self.krylov(self._b32, self._x32, True, tol, maxiter, method, restart, check_every)
+            its, rel_in = self.krylov(self._b32, self._x32, True, tol, inner_max, method, restart, check_every)
             total += its
+            rel = rel_in
             L.check(lib.gm_axpy(n, 1.0, L.ptr(self._x32), 1, L.ptr(self.delta), _stream()), 'gm_axpy')
         self.stats['krylov_iters'].append(total)
         self.stats['krylov_relres'].append(rel)
