@@ -282,6 +282,7 @@ def main():
         phases = s.engine.timers.summary()
         s.engine.timers = None
         res[form] = dict(seconds=t1 - t0, newton_iters=iters, krylov_iters=kry,
+                         spmv_autotune=s.engine.stats.get('spmv_tune'),
                          ms_per_step=(t1 - t0) * 1e3 / args.steps, newton_per_s=iters / (t1 - t0),
                          phases_ms={k: round(v, 3) for k, v in phases.items()},
                          last_residuals=['%.2e' % v for v in s.residual_history])

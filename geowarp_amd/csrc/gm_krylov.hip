@@ -23,10 +23,10 @@
 #define RED_T 256
 
 // ------------------------------------------------------------------ SpMV
-template <int NB, int G, typename TM, typename TX, typename TY>
-__global__ void __launch_bounds__(SPMV_T) k_spmv(int n_rows, const int* row_ptr, const int* col_idx, const TM* vals,
-                                                 const TX* x, TY* y, const TY* w1, const TY* w2, double* part) {
-  const int rows_per_blk = SPMV_T / G;
+template <int NB, int G, typename TM, typename TX, typename TY, int T = SPMV_T>
+__global__ void __launch_bounds__(T) k_spmv(int n_rows, const int* row_ptr, const int* col_idx, const TM* vals,
+                                            const TX* x, TY* y, const TY* w1, const TY* w2, double* part) {
+  const int rows_per_blk = T / G;
   int lr = threadIdx.x / G;
   int lane = threadIdx.x % G;
   int row = blockIdx.x * rows_per_blk + lr;
@@ -62,11 +62,11 @@ __global__ void __launch_bounds__(SPMV_T) k_spmv(int n_rows, const int* row_ptr,
     }
   }
   if (part) {
-    __shared__ double sh1[SPMV_T], sh2[SPMV_T];
+    __shared__ double sh1[T], sh2[T];
     sh1[threadIdx.x] = d1;
     sh2[threadIdx.x] = d2;
     __syncthreads();
-    for (int o = SPMV_T / 2; o > 0; o >>= 1) {
+    for (int o = T / 2; o > 0; o >>= 1) {
       if (threadIdx.x < o) {
         sh1[threadIdx.x] += sh1[threadIdx.x + o];
         sh2[threadIdx.x] += sh2[threadIdx.x + o];
@@ -80,12 +80,34 @@ __global__ void __launch_bounds__(SPMV_T) k_spmv(int n_rows, const int* row_ptr,
   }
 }
 
+// Workgroup size of the SpMV per (block size, matrix precision), chosen by
+// gm_spmv_autotune (256 until tuned).  Launches with fused dot products keep
+// 256 threads so their per-workgroup partials, and hence their round-off, do
+// not depend on the tuning; without them the workgroup size does not change
+// any result bit (each row's lanes and reduction order stay the same).
+static int g_spmv_threads[5][2] = {{256, 256}, {256, 256}, {256, 256}, {256, 256}, {256, 256}};
+
 template <int NB, typename TM, typename TX, typename TY>
 static void launch_spmv(int n_rows, const int* rp, const int* ci, const TM* v, const TX* x, TY* y, const TY* w1,
-                        const TY* w2, double* part, hipStream_t s) {
+                        const TY* w2, double* part, hipStream_t s, int threads = 0) {
   const int G = (NB >= 3) ? 8 : 4;
-  int blk = gm_blocks(n_rows, SPMV_T / G);
-  k_spmv<NB, G, TM, TX, TY><<<blk, SPMV_T, 0, s>>>(n_rows, rp, ci, v, x, y, w1, w2, part);
+  int t = part ? SPMV_T : (threads ? threads : g_spmv_threads[NB][sizeof(TM) == 4 ? 1 : 0]);
+  switch (t) {
+    case 64:
+      k_spmv<NB, G, TM, TX, TY, 64><<<gm_blocks(n_rows, 64 / G), 64, 0, s>>>(n_rows, rp, ci, v, x, y, w1, w2, part);
+      break;
+    case 128:
+      k_spmv<NB, G, TM, TX, TY, 128><<<gm_blocks(n_rows, 128 / G), 128, 0, s>>>(n_rows, rp, ci, v, x, y, w1, w2,
+                                                                                 part);
+      break;
+    case 512:
+      k_spmv<NB, G, TM, TX, TY, 512><<<gm_blocks(n_rows, 512 / G), 512, 0, s>>>(n_rows, rp, ci, v, x, y, w1, w2,
+                                                                                 part);
+      break;
+    default:
+      k_spmv<NB, G, TM, TX, TY, 256><<<gm_blocks(n_rows, 256 / G), 256, 0, s>>>(n_rows, rp, ci, v, x, y, w1, w2,
+                                                                                 part);
+  }
 }
 
 static int spmv_blocks(int n_rows, int nb) { return gm_blocks(n_rows, SPMV_T / ((nb >= 3) ? 8 : 4)); }
@@ -1589,4 +1611,51 @@ extern "C" int gm_row_scale(int nb, int fp32, int n_rows, const int* row_ptr, co
 #undef GM_RS
   gm::set_error("gm_row_scale: unsupported block size %d", nb);
   return -1;
+}
+
+// ================================================================== SpMV autotuner
+// Times the SpMV on a given matrix for each workgroup size (64, 128, 256, 512
+// threads; `reps` launches each, HIP events on the stream), stores the fastest
+// for later launches with this block size and precision, and returns it with
+// the four average times in milliseconds.
+extern "C" int gm_spmv_autotune(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx,
+                                const void* vals, const void* x, void* y, int reps, int* best, float* ms,
+                                hipStream_t s) {
+  GM_REQUIRE(nb >= 1 && nb <= 4 && row_ptr && col_idx && vals && x && y && best && ms && reps >= 1,
+             "gm_spmv_autotune: bad argument");
+  const int cand[4] = {64, 128, 256, 512};
+  hipEvent_t e0, e1;
+  GM_HIP(hipEventCreate(&e0));
+  GM_HIP(hipEventCreate(&e1));
+  int bi = -1;
+  for (int c = 0; c < 4; ++c) {
+    auto launch = [&]() {
+#define GM_TUNE(NB)                                                                                              \
+  if (nb == NB) {                                                                                                \
+    if (fp32)                                                                                                    \
+      launch_spmv<NB, float, float, float>(n_rows, row_ptr, col_idx, (const float*)vals, (const float*)x,       \
+                                           (float*)y, nullptr, nullptr, nullptr, s, cand[c]);                  \
+    else                                                                                                         \
+      launch_spmv<NB, double, double, double>(n_rows, row_ptr, col_idx, (const double*)vals, (const double*)x,  \
+                                              (double*)y, nullptr, nullptr, nullptr, s, cand[c]);              \
+  }
+      GM_TUNE(1) GM_TUNE(2) GM_TUNE(3) GM_TUNE(4)
+#undef GM_TUNE
+    };
+    launch();  // warm-up
+    GM_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) launch();
+    GM_HIP(hipEventRecord(e1, s));
+    GM_HIP(hipEventSynchronize(e1));
+    GM_LAUNCH_CHECK();
+    float t = 0.0f;
+    GM_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms[c] = t / reps;
+    if (bi < 0 || ms[c] < ms[bi]) bi = c;
+  }
+  GM_HIP(hipEventDestroy(e0));
+  GM_HIP(hipEventDestroy(e1));
+  *best = cand[bi];
+  g_spmv_threads[nb][fp32 ? 1 : 0] = cand[bi];
+  return 0;
 }

@@ -103,6 +103,8 @@ class _QuasiStaticBase:
         self.equilibrate = it.get('equilibrate', False)
         # backtracking line search on the Newton correction (the reference takes full steps)
         self.line_search = it.get('line_search', False)
+        # time the SpMV workgroup sizes on the first Jacobian and keep the fastest
+        self.autotune = it.get('autotune', True)
         self.line_search_steps = []
         # multigrid V-cycle options (krylov='mg'): smoothing sweeps, damping, coarsest-level sweeps
         mg_opts = {k: it[k2] for k, k2 in (('nu_pre', 'mg_nu'), ('nu_post', 'mg_nu'), ('omega', 'mg_omega'),
@@ -240,6 +242,8 @@ class _QuasiStaticBase:
             if tm is not None:
                 tm.mark('node_residual')
             eng.jacobian(x, lp)
+            if self.autotune and eng.stats.get('spmv_tune') is None:
+                eng.autotune_spmv()
             if tm is not None:
                 tm.mark('node_jacobian')
             nr = math.sqrt(max(eng.scal[0].item(), 0.0))

@@ -254,6 +254,23 @@ class Engine:
             self._ws[key] = [Multigrid(self, fp32, **self.mg_options), -1]
         return self._ws[key]
 
+    def autotune_spmv(self, reps=10):
+        """Pick the SpMV workgroup size on the current Jacobian (once per engine
+        and precision); the timings are kept in stats['spmv_tune']."""
+        fp32 = self.precision == 'mixed'
+        n = self.n_nodes * self.nb
+        dt = torch.float32 if fp32 else torch.float64
+        x = torch.ones(n, dtype=dt, device=self.dev)
+        y = torch.empty(n, dtype=dt, device=self.dev)
+        best = L.c_int(0)
+        ms = (L.ctypes.c_float * 4)()
+        L.check(L.lib().gm_spmv_autotune(self.nb, int(fp32), self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
+                                         L.ptr(self.vals), L.ptr(x), L.ptr(y), reps, L.ctypes.byref(best), ms,
+                                         _stream()), 'gm_spmv_autotune')
+        self.stats['spmv_tune'] = dict(threads=best.value, ms={t: round(v, 5) for t, v in
+                                                               zip((64, 128, 256, 512), list(ms))}, nnzb=self.nnzb)
+        return best.value
+
     def jacobian(self, x, lp):
         fp32 = self.precision == 'mixed'
         self.jac_version += 1

@@ -265,6 +265,14 @@ int gm_gmres_up(int dim, int nb, int fp32, int n_rows, const int* row_ptr, const
 int gm_row_scale(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, void* vals,
                  const double* rhs_in, double* rhs_out, hipStream_t stream);
 
+/* SpMV autotuner (no reference counterpart): times gm_bsr_spmv's kernel on the
+   given matrix for workgroups of 64, 128, 256 and 512 threads (reps launches
+   each, on the stream), keeps the fastest for later SpMVs with this block size
+   and precision (process-wide) and returns it in *best with the four average
+   times (ms[4], milliseconds).  The choice never changes a result bit. */
+int gm_spmv_autotune(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
+                     const void* x, void* y, int reps, int* best, float* ms, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
