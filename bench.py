@@ -30,6 +30,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -117,7 +119,8 @@ def spmv_roofline(s, reps=50):
     cy = torch.zeros(1 << 25, dtype=torch.float64, device=eng.dev)
     L.check(lib.gm_axpy(cx.numel(), 1.0, L.ptr(cx), 0, L.ptr(cy), stream), 'gm_axpy')
     torch.cuda.synchronize()
-    return dict(kernel='k_spmv<%d,8,%s>' % (nb, 'float' if fp32 else 'double'), avg_us=avg_ms * 1e3,
+    lanes = 16 if nb >= 4 else (8 if nb == 3 else 4)
+    return dict(kernel='k_spmv<%d,%d,%s>' % (nb, lanes, 'float' if fp32 else 'double'), avg_us=avg_ms * 1e3,
                 bytes_per_launch=bytes_, nnzb=eng.nnzb, achieved_GBs=achieved)
 
 
@@ -132,7 +135,9 @@ def adjoint_bench(steps, total=30, every=5, n_specimens=64, krylov='mg'):
     from geowarp_amd import problems as P
     from geowarp_amd.adjoint import LoadPathAdjoint, LoadPathFunction
     W, phi, coh = P.specimen_counts(n_specimens, seed=0)
-    cfg = P.specimens2d(W, phi, coh, total_steps=total, strain=0.03, tol=1e-8, n_iter=12)
+    # Newton to 1e-12 relative residual: the finite-difference check below resolves loss changes of
+    # ~1e-7 relative, which needs forces converged well beyond that
+    cfg = P.specimens2d(W, phi, coh, total_steps=total, strain=0.03, tol=1e-12, n_iter=14)
     cfg['iteration_dict']['krylov'] = krylov
     s = P.build_solver(gs, cfg, device='cuda:0')
     cfg['set_boundary_dofs'](s.dofStruct.boundary_flag_array, s.n_grid_x, s.n_nodes)
@@ -152,14 +157,32 @@ def adjoint_bench(steps, total=30, every=5, n_specimens=64, krylov='mg'):
     loss.backward()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    unconverged = list(adj.stats['unconverged_steps'])
+    kits = adj.stats['krylov_iters']
+    # central finite difference (h = 1e-6 deg) of one component: the largest specimen's friction angle
+    i = int(np.argmax(W))
+    h = 1e-6
+
+    def loss_at(dphi):
+        p = phi + 2.0
+        p[i] += dphi
+        with torch.no_grad():
+            ff = LoadPathFunction.apply(torch.tensor(p, device=dev), torch.tensor(coh + 3.0, device=dev), adj)
+        return float(((ff - f_obs) ** 2).sum())
+
+    fd = (loss_at(h) - loss_at(-h)) / (2 * h)
+    ad = float(phi_t.grad[i])
     return dict(workload='%d 2D Drucker-Prager specimens (seed 0, %d particles, %d-%d cells wide), load steps '
                          '0-%d of %d, checkpoint every %d, loss = sum of squared reaction-force misfits'
                          % (n_specimens, s.n_particles, int(W.min()), int(W.max()), steps - 1, total, every),
                 seconds_forward=round(t1 - t0, 3), seconds_adjoint=round(t2 - t1, 3),
                 grads_per_s=round(1.0 / (t2 - t0), 4), gradient_components=2 * n_specimens,
                 forward_steps=adj.stats['forward_steps'], adjoint_steps=adj.stats['adjoint_steps'],
-                transposed_solve_iters=adj.stats['krylov_iters'],
-                grad_norm=dict(phi=float(phi_t.grad.norm()), coh=float(coh_t.grad.norm())))
+                transposed_solve_iters=dict(min=min(kits), max=max(kits), mean=round(sum(kits) / len(kits), 1)),
+                unconverged_forward_steps=unconverged,
+                grad_norm=dict(phi=float(phi_t.grad.norm()), coh=float(coh_t.grad.norm())),
+                fd_check=dict(component='dloss/dphi of specimen %d (%d cells wide)' % (i, int(W[i])), h=h,
+                              adjoint=ad, central_fd=fd, rel_err=abs(ad - fd) / max(abs(fd), 1e-300)))
 
 
 def c2_bench(steps):

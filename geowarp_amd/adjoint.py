@@ -77,13 +77,15 @@ class LoadPathAdjoint:
         self.adjoint_tol = adjoint_tol
         # initial state of the load path (default: the solver's current state)
         self.initial = initial if initial is not None else self._snapshot()
-        self.stats = dict(forward_steps=0, adjoint_steps=0, krylov_iters=[])
+        self.stats = dict(forward_steps=0, adjoint_steps=0, krylov_iters=[], unconverged_steps=[])
 
     # ------------------------------------------------------------------ state
     def _snapshot(self):
         s = self.s
+        # the boundary flags are state too: a prescribed-increment hook may change them every step
         return dict(x=s.x_particles.clone(), lp=s.GIMP_lp.clone(), F=s.deformation_gradient_total_old.clone(),
-                    b=s.left_Cauchy_Green_old.clone(), a=s.plastic_strain_old.clone())
+                    b=s.left_Cauchy_Green_old.clone(), a=s.plastic_strain_old.clone(),
+                    bc=s.dofStruct.boundary_flag_array.clone())
 
     def _restore(self, st):
         s = self.s
@@ -92,6 +94,7 @@ class LoadPathAdjoint:
         s.deformation_gradient_total_old.copy_(st['F'])
         s.left_Cauchy_Green_old.copy_(st['b'])
         s.plastic_strain_old.copy_(st['a'])
+        s.dofStruct.boundary_flag_array.copy_(st['bc'])
 
     def _local_args(self, step, u, cot=None, cot_scale=None, bars=None):
         s = self.s
@@ -150,6 +153,9 @@ class LoadPathAdjoint:
         s = self.s
         s.prepare_step(step, self.total_steps)
         s.newton_iter(step, self.total_steps)
+        if not getattr(s, 'newton_converged', True):
+            # the sensitivities assume R(u_k) = 0; record the steps where Newton stopped short
+            self.stats['unconverged_steps'].append((step, s.residual_history[-1]))
         u = s._inc.clone()
         f = self._reaction(step)
         s.finish_step()

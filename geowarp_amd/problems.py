@@ -288,7 +288,9 @@ def specimens2d(widths, phi, coh, dx=0.01, total_steps=30, strain=0.03, E=1.0e4,
     cells apart.  Each specimen sits in a box with frictionless walls (x fixed
     on its side node lines), a fixed base and a top platen that moves down by
     strain * height over total_steps load steps (prescribed y increments on the
-    node rows at and above its current top).  Per-particle friction angle and
+    node rows at and above the row nearest to its current top, so that the
+    platen keeps a support weight of at least half a cell on the top
+    particles).  Per-particle friction angle and
     cohesion come from the specimen (solver.friction_angle_array /
     cohesion_array); `groups` maps particles to specimens."""
     widths = [int(w) for w in widths]
@@ -336,13 +338,14 @@ def specimens2d(widths, phi, coh, dx=0.01, total_steps=30, strain=0.03, E=1.0e4,
         top = _to_np(xp[:, 1] + lp[:, 1])
         tops = np.full(len(widths), -np.inf)
         np.maximum.at(tops, groups, top)
-        rows = np.ceil(tops / dx_ - 1e-8).astype(int)
+        rows = np.floor(tops / dx_ + 0.5).astype(int)  # node row nearest to the top surface
         iy = np.arange(ny + 1)[:, None]
         col_row = np.where(owner >= 0, rows[np.maximum(owner, 0)], ny + 2)
         m = (iy >= col_row[None, :]) & (owner[None, :] >= 0)
         val = np.where(owner >= 0, dy_step[np.maximum(owner, 0)] * total_steps / float(total_steps_), 0.0)
         by = _view2d(boundary, 1, n_nodes, nx1)
         py = _view2d(pres, 1, n_nodes, nx1)
+        by[pad_lo + 1:, :] = False  # the platen rows of earlier steps (the base rows stay fixed)
         by[_like(by, m)] = True
         py[_like(py, m)] = _like(py, np.broadcast_to(val[None, :], m.shape)[m])
 

@@ -275,6 +275,7 @@ class _QuasiStaticBase:
             if crit < self.tol:
                 break
         self.newton_iterations = it + 1
+        self.newton_converged = bool(self.residual_history) and crit < self.tol
 
     def _line_search(self, args, u_new, nr, after_residual, max_halvings=6):
         """Backtracking on the Newton correction (not in the reference, which

@@ -136,3 +136,28 @@ def test_batched_gradients_match_device_fd_and_checkpointing():
         print(i, 'phi', grads[0][0][i], fd_phi, 'coh', grads[0][1][i], fd_coh)
         assert abs(grads[0][0][i] - fd_phi) <= 1e-4 * abs(fd_phi)
         assert abs(grads[0][1][i] - fd_coh) <= 1e-4 * abs(fd_coh)
+
+
+@pytest.mark.gpu
+def test_forward_repeats_from_the_initial_state():
+    """A load path whose platen rows move down (20 % strain) gives the same forces
+    when it is run again after a run with other parameters: the snapshot of the
+    initial state includes the boundary flags that the platen hook rewrites."""
+    from geowarp_amd.adjoint import LoadPathAdjoint, LoadPathFunction
+    cfg = P.specimens2d([3, 4], [30.0, 36.0], [5.0, 12.0], total_steps=8, strain=0.2, tol=1e-12)
+    s, g = _solver(cfg)
+    adj = LoadPathAdjoint(s, 8, groups=g, checkpoint_every=3)
+    dev = s.device
+    bc0 = s.dofStruct.boundary_flag_array.clone()
+    runs = []
+    for dphi in (0.0, 3.0, 0.0):
+        with torch.no_grad():
+            f = LoadPathFunction.apply(torch.tensor([30.0 + dphi, 36.0], device=dev),
+                                       torch.tensor([5.0, 12.0], device=dev), adj)
+        runs.append(f.cpu().numpy())
+        print('dphi', dphi, 'forces', runs[-1][:, 0], 'unconverged', adj.stats['unconverged_steps'])
+    assert adj.stats['unconverged_steps'] == []
+    assert not np.array_equal(runs[0], runs[1])
+    np.testing.assert_array_equal(runs[0], runs[2])
+    # the platen rows did move: the flags at the end of the path differ from the initial ones
+    assert not torch.equal(s.dofStruct.boundary_flag_array, bc0)
