@@ -199,19 +199,29 @@ def c2_bench(steps):
         s = P.build_solver(gs, cfg, device='cuda:0')
         cfg['set_boundary_dofs'](s.dofStruct.boundary_flag_array, s.n_grid_x, s.n_nodes)
         s.line_search = True
+        from geowarp_amd._lib import GMError
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         iters = 0
-        for k in range(steps):
-            s.advance_one_step(k, cfg['total_steps'])
-            iters += s.newton_iterations
+        err = None
+        try:
+            for k in range(steps):
+                s.advance_one_step(k, cfg['total_steps'])
+                iters += s.newton_iterations
+        except GMError as e:  # a diverged form is reported, not fatal to the benchmark
+            err = str(e)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         hist = s.residual_history
-        conv = bool(hist and hist[0] > 0 and hist[-1] / hist[0] < s.tol)
+        conv = err is None and bool(hist and hist[0] > 0 and hist[-1] / hist[0] < s.tol)
         out[form] = dict(seconds=round(dt, 4), newton_iters=iters, newton_per_s=round(iters / dt, 4),
                          converged=conv, last_residuals=['%.2e' % v for v in hist])
+        if err is not None:
+            out[form]['error'] = err
         solvers[form] = s
+    if 'error' in out['fp64'] or 'error' in out['mixed']:
+        out['workload'] = '2D J2 bar 256x1024 cells, load steps 0-%d' % (steps - 1)
+        return out
     a, b = solvers['mixed'], solvers['fp64']
     du = float(torch.linalg.norm((a.x_particles - a.x0) - (b.x_particles - b.x0)) / torch.linalg.norm(b.x_particles - b.x0))
     ds = float(torch.linalg.norm(a.particle_Cauchy_stress_array - b.particle_Cauchy_stress_array)

@@ -65,7 +65,7 @@ class gm_mg_level(ctypes.Structure):
     _fields_ = [('nx', c_int), ('ny', c_int), ('nz', c_int), ('n_nodes', c_int), ('nnzb', c_int),
                 ('row_ptr', c_void_p), ('col_idx', c_void_p), ('row_mask', c_void_p), ('vals', c_void_p),
                 ('dinv', c_void_p), ('freem', c_void_p), ('x', c_void_p), ('b', c_void_p), ('r', c_void_p),
-                ('dense_inv', c_void_p), ('gamma', c_int)]
+                ('dense_inv', c_void_p), ('gamma', c_int), ('vals_lo', c_void_p), ('row_scale', c_void_p)]
 
 
 class gm_krylov_ws(ctypes.Structure):

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "gm_common.h"
@@ -87,12 +88,14 @@ __global__ void __launch_bounds__(T) k_spmv(int n_rows, const int* row_ptr, cons
 // any result bit (each row's lanes and reduction order stay the same).
 static int g_spmv_threads[5][2] = {{256, 256}, {256, 256}, {256, 256}, {256, 256}, {256, 256}};
 
+// Lanes per block row: 3D u-p rows hold 27-125 blocks (16 lanes measured 4-6 % faster than 8
+// on the 64x64x32-cell indentation Jacobian in both precisions); 2D rows hold at most 25.
+static constexpr int spmv_lanes(int nb) { return nb >= 4 ? 16 : (nb == 3 ? 8 : 4); }
+
 template <int NB, typename TM, typename TX, typename TY>
 static void launch_spmv(int n_rows, const int* rp, const int* ci, const TM* v, const TX* x, TY* y, const TY* w1,
                         const TY* w2, double* part, hipStream_t s, int threads = 0) {
-  // lanes per block row: 3D u-p rows hold 27-125 blocks (16 lanes measured 4-6 % faster than 8
-  // on the 64x64x32-cell indentation Jacobian in both precisions); 2D rows hold at most 25
-  const int G = (NB >= 4) ? 16 : (NB == 3 ? 8 : 4);
+  const int G = spmv_lanes(NB);
   int t = part ? SPMV_T : (threads ? threads : g_spmv_threads[NB][sizeof(TM) == 4 ? 1 : 0]);
   switch (t) {
     case 64:
@@ -112,7 +115,103 @@ static void launch_spmv(int n_rows, const int* rp, const int* ci, const TM* v, c
   }
 }
 
-static int spmv_blocks(int n_rows, int nb) { return gm_blocks(n_rows, SPMV_T / ((nb >= 3) ? 8 : 4)); }
+// ------------------------------------------------------------------ FP16 smoother operator
+// An FP32 level matrix stored again in FP16, each (scalar row, column dof)
+// sub-row of a block row scaled by its largest magnitude, so that every stored
+// value lies in [-1, 1] and the small pressure-pressure entries of u-p systems
+// keep their own range:
+//     A[k][r][c] ~ lo[k][r][c] * scale[(row * NB + r) * NB + c].
+// The V-cycle's smoothing and residual products read it (half the matrix
+// bytes of FP32); the Krylov operator itself stays FP32.
+typedef _Float16 gm_half;
+
+template <int NB, int G, int T, typename TM>
+__global__ void __launch_bounds__(T) k_lo_convert(int n_rows, const int* row_ptr, const TM* vals, gm_half* lo,
+                                                  float* scale) {
+  const int rows_per_blk = T / G;
+  int row = blockIdx.x * rows_per_blk + threadIdx.x / G;
+  int lane = threadIdx.x % G;
+  float mx[NB * NB];
+#pragma unroll
+  for (int q = 0; q < NB * NB; ++q) mx[q] = 0.0f;
+  int s = 0, e = 0;
+  if (row < n_rows) {
+    s = row_ptr[row];
+    e = row_ptr[row + 1];
+    for (int k = s + lane; k < e; k += G)
+#pragma unroll
+      for (int q = 0; q < NB * NB; ++q) mx[q] = fmaxf(mx[q], fabsf((float)vals[(long)k * NB * NB + q]));
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int q = 0; q < NB * NB; ++q) mx[q] = fmaxf(mx[q], __shfl_xor(mx[q], off, G));
+  if (row >= n_rows) return;
+  float inv[NB * NB];
+#pragma unroll
+  for (int q = 0; q < NB * NB; ++q) {
+    float sc = (mx[q] > 0.0f && mx[q] < 3.0e38f) ? mx[q] : 1.0f;
+    inv[q] = 1.0f / sc;
+    if (lane == 0) scale[(long)row * NB * NB + q] = sc;
+  }
+  for (int k = s + lane; k < e; k += G)
+#pragma unroll
+    for (int q = 0; q < NB * NB; ++q) lo[(long)k * NB * NB + q] = (gm_half)((float)vals[(long)k * NB * NB + q] * inv[q]);
+}
+
+template <int NB, int G, int T>
+__global__ void __launch_bounds__(T) k_spmv_lo(int n_rows, const int* row_ptr, const int* col_idx,
+                                               const gm_half* lo, const float* scale, const float* x, float* y) {
+  const int rows_per_blk = T / G;
+  int row = blockIdx.x * rows_per_blk + threadIdx.x / G;
+  int lane = threadIdx.x % G;
+  float acc[NB * NB];  // per (r, c): sum over the row's blocks of lo[r][c] x[c]
+#pragma unroll
+  for (int q = 0; q < NB * NB; ++q) acc[q] = 0.0f;
+  if (row < n_rows) {
+    int s = row_ptr[row], e = row_ptr[row + 1];
+    for (int k = s + lane; k < e; k += G) {
+      int c = col_idx[k];
+      const gm_half* blk = lo + (long)k * NB * NB;
+      float xv[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) xv[j] = x[(long)c * NB + j];
+#pragma unroll
+      for (int r = 0; r < NB; ++r)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[r * NB + j] += (float)blk[r * NB + j] * xv[j];
+    }
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int q = 0; q < NB * NB; ++q) acc[q] += __shfl_xor(acc[q], off, G);
+  if (row < n_rows && lane == 0)
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      float v = 0.0f;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) v += scale[((long)row * NB + r) * NB + j] * acc[r * NB + j];
+      y[(long)row * NB + r] = v;
+    }
+}
+
+template <int NB, typename TM>
+static void launch_lo_convert(int n_rows, const int* rp, const TM* vals, gm_half* lo, float* scale,
+                              hipStream_t s) {
+  constexpr int G = spmv_lanes(NB);
+  k_lo_convert<NB, G, 256, TM><<<gm_blocks(n_rows, 256 / G), 256, 0, s>>>(n_rows, rp, vals, lo, scale);
+}
+
+template <int NB>
+static void launch_spmv_lo(int n_rows, const int* rp, const int* ci, const gm_half* lo, const float* scale,
+                           const float* x, float* y, hipStream_t s) {
+  constexpr int G = spmv_lanes(NB);
+  k_spmv_lo<NB, G, 256><<<gm_blocks(n_rows, 256 / G), 256, 0, s>>>(n_rows, rp, ci, lo, scale, x, y);
+}
+
+// workgroups of a launch with fused dot products (SPMV_T threads): the partial-sum count
+static int spmv_blocks(int n_rows, int nb) { return gm_blocks(n_rows, SPMV_T / spmv_lanes(nb)); }
 
 // ------------------------------------------------------------------ reductions
 // Sum `n` partial pairs (stride 2) deterministically into out[0], out[1].
@@ -172,8 +271,8 @@ static int dot_blocks(long n) {
 }
 
 // ------------------------------------------------------------------ block Jacobi
-template <int NB, typename TM>
-__global__ void k_diag_inv(int n_rows, const int* row_ptr, const int* col_idx, const TM* vals, TM* dinv) {
+template <int NB, typename TM, typename TD = TM>
+__global__ void k_diag_inv(int n_rows, const int* row_ptr, const int* col_idx, const TM* vals, TD* dinv) {
   int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n_rows) return;
   int s = row_ptr[row], e = row_ptr[row + 1];
@@ -227,7 +326,7 @@ __global__ void k_diag_inv(int n_rows, const int* row_ptr, const int* col_idx, c
 #pragma unroll
   for (int i = 0; i < NB; ++i)
 #pragma unroll
-    for (int j = 0; j < NB; ++j) dinv[(long)row * NB * NB + i * NB + j] = (TM)M[i][NB + j];
+    for (int j = 0; j < NB; ++j) dinv[(long)row * NB * NB + i * NB + j] = (TD)M[i][NB + j];
 }
 
 template <int NB, typename TM, typename TV>
@@ -777,20 +876,36 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
       int prc = prec.apply(vj, zj, st);
       if (prc) return prc;
       launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, zj, w, nullptr, nullptr, nullptr, st);
-      // CGS2
-      for (int pass = 0; pass < 2; ++pass) {
-        k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, st>>>(n, j + 1, V, ldv, w, nchunk, part);
-        k_multidot_fin<<<j + 1, RED_T, 0, st>>>(nchunk, part, dh + pass * (m + 2));
-        k_multiaxpy<TV><<<vb, RED_T, 0, st>>>(n, j + 1, V, ldv, dh + pass * (m + 2), w);
-      }
+      // classical Gram-Schmidt with selective reorthogonalisation (Daniel, Gragg,
+      // Kaufman & Stewart 1976): a second pass only when the first one removed
+      // more than half of ||w||^2.  dh: [0, m+2) pass-1 coefficients, [m+2, 2m+4)
+      // pass-2 coefficients, [2m+4] ||w||^2 after the passes, [2m+5] before them.
+      const int nsl = 2 * (m + 2);
       k_dot2<TV><<<vb, RED_T, 0, st>>>(n, w, w, nullptr, nullptr, part);
-      k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + 2 * (m + 2), nullptr);
+      k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + nsl + 1, nullptr);
+      k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, st>>>(n, j + 1, V, ldv, w, nchunk, part);
+      k_multidot_fin<<<j + 1, RED_T, 0, st>>>(nchunk, part, dh);
+      k_multiaxpy<TV><<<vb, RED_T, 0, st>>>(n, j + 1, V, ldv, dh, w);
+      k_dot2<TV><<<vb, RED_T, 0, st>>>(n, w, w, nullptr, nullptr, part);
+      k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + nsl, nullptr);
       GM_LAUNCH_CHECK();
       std::vector<double> hh(3 * (m + 2));
       GM_HIP(hipMemcpyAsync(hh.data(), dh, sizeof(double) * 3 * (m + 2), hipMemcpyDeviceToHost, st));
       GM_HIP(hipStreamSynchronize(st));
-      for (int i = 0; i <= j; ++i) hcol[i] = hh[i] + hh[(m + 2) + i];
-      double hn = sqrt(fmax(hh[2 * (m + 2)], 0.0));
+      for (int i = 0; i <= j; ++i) hcol[i] = hh[i];
+      if (!(hh[nsl] >= 0.5 * hh[nsl + 1])) {  // also taken when a norm is NaN
+        k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, st>>>(n, j + 1, V, ldv, w, nchunk, part);
+        k_multidot_fin<<<j + 1, RED_T, 0, st>>>(nchunk, part, dh + (m + 2));
+        k_multiaxpy<TV><<<vb, RED_T, 0, st>>>(n, j + 1, V, ldv, dh + (m + 2), w);
+        k_dot2<TV><<<vb, RED_T, 0, st>>>(n, w, w, nullptr, nullptr, part);
+        k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + nsl, nullptr);
+        GM_LAUNCH_CHECK();
+        GM_HIP(hipMemcpyAsync(hh.data() + (m + 2), dh + (m + 2), sizeof(double) * (m + 3), hipMemcpyDeviceToHost,
+                              st));
+        GM_HIP(hipStreamSynchronize(st));
+        for (int i = 0; i <= j; ++i) hcol[i] += hh[(m + 2) + i];
+      }
+      double hn = sqrt(fmax(hh[nsl], 0.0));
       if (!(hn == hn) || hn > 1e300 || !(hcol[j] == hcol[j])) {
         gm::set_error("gmres: Arnoldi vector became non-finite after %d iterations (preconditioner overflow)",
                       total);
@@ -952,7 +1067,7 @@ __global__ void k_free(Dims f, Dims c, int n_coarse, const uint8_t* freem_f, uin
 // nodes J with J_axis = floor((i_axis - 2) / 2) + t_axis.
 template <int DIM, int NB, typename T>
 __global__ void k_ap(Dims f, Dims c, int n_fine, const int* rp, const uint64_t* mask, const T* vals,
-                     const uint8_t* freem_f, T* AP) {
+                     const uint8_t* freem_f, double* AP) {
   const int TT = (DIM == 2) ? 16 : 64;
   long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (gid >= (long)n_fine * TT) return;
@@ -996,15 +1111,15 @@ __global__ void k_ap(Dims f, Dims c, int n_fine, const int* rp, const uint64_t* 
           }
         }
   }
-  T* out = AP + gid * NB * NB;
+  double* out = AP + gid * NB * NB;
 #pragma unroll
-  for (int k = 0; k < NB * NB; ++k) out[k] = (T)acc[k];
+  for (int k = 0; k < NB * NB; ++k) out[k] = acc[k];
 }
 
 // A_c[I, J] = sum_i P[i, I] AP[i, J] over free fine rows; identity on coarse dofs without free fine dofs.
-template <int DIM, int NB, typename T>
-__global__ void k_rap(Dims f, Dims c, int n_coarse, const int* rp_c, const uint64_t* mask_c, T* vals_c,
-                      const uint8_t* freem_f, const uint8_t* freem_c, const T* AP) {
+template <int DIM, int NB>
+__global__ void k_rap(Dims f, Dims c, int n_coarse, const int* rp_c, const uint64_t* mask_c, double* vals_c,
+                      const uint8_t* freem_f, const uint8_t* freem_c, const double* AP) {
   const int TT = (DIM == 2) ? 16 : 64;
   const int W = (DIM == 2) ? 25 : 125;
   long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -1037,7 +1152,7 @@ __global__ void k_rap(Dims f, Dims c, int n_coarse, const int* rp_c, const uint6
         if (!in) continue;
         double w = (dx == 0 ? 1.0 : 0.5) * (dy == 0 ? 1.0 : 0.5) * ((DIM == 3 && dz != 0) ? 0.5 : 1.0);
         int i = node_of<DIM>(a, f.n[0], f.n[1]);
-        const T* blk = AP + ((long)i * TT + t) * NB * NB;
+        const double* blk = AP + ((long)i * TT + t) * NB * NB;
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
           if (!freem_f[(long)i * NB + r]) continue;
@@ -1049,9 +1164,9 @@ __global__ void k_rap(Dims f, Dims c, int n_coarse, const int* rp_c, const uint6
 #pragma unroll
     for (int r = 0; r < NB; ++r)
       if (!freem_c[(long)I * NB + r]) acc[r * NB + r] = 1.0;
-  T* out = vals_c + (long)q * NB * NB;
+  double* out = vals_c + (long)q * NB * NB;
 #pragma unroll
-  for (int k = 0; k < NB * NB; ++k) out[k] = (T)acc[k];
+  for (int k = 0; k < NB * NB; ++k) out[k] = acc[k];
 }
 
 // b_c = P^T r_f (free fine dofs only)
@@ -1150,7 +1265,7 @@ __global__ void k_jac_update(int n_rows, const T* dinv, const T* b, const T* ax,
 
 // x = Minv b for the coarsest level's dense inverse (one row per thread)
 template <typename T>
-__global__ void k_dense_mv(int n, const T* Minv, const T* b, T* x) {
+__global__ void k_dense_mv(int n, const double* Minv, const T* b, T* x) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
@@ -1184,25 +1299,42 @@ struct Cycle {
   const gm_mg_level* L;
 };
 
+// y = A x on level l, from the FP16 copy where the level has one (FP32 hierarchies);
+// level 0 holds the Jacobian in the solve precision, coarser levels are FP64
+template <int NB, typename T>
+static void level_spmv(const gm_mg_level& lv, int l, const T* x, T* y, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (lv.vals_lo) {
+      launch_spmv_lo<NB>(lv.n_nodes, lv.row_ptr, lv.col_idx, (const gm_half*)lv.vals_lo, lv.row_scale, x, y, st);
+      return;
+    }
+  }
+  if (l == 0)
+    launch_spmv<NB, T, T, T>(lv.n_nodes, lv.row_ptr, lv.col_idx, (const T*)lv.vals, x, y, nullptr, nullptr,
+                             nullptr, st);
+  else
+    launch_spmv<NB, double, T, T>(lv.n_nodes, lv.row_ptr, lv.col_idx, (const double*)lv.vals, x, y, nullptr,
+                                  nullptr, nullptr, st);
+}
+
 template <int DIM, int NB, typename T>
 static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
   const gm_mg_level& lv = cy.L[l];
   int n = lv.n_nodes;
   int rb = gm_blocks(n, 256);
   const T* dinv = (const T*)lv.dinv;
-  const T* vals = (const T*)lv.vals;
   T* r = (T*)lv.r;
   bool coarsest = (l == cy.n_levels - 1);
   if (coarsest && lv.dense_inv) {
     int nd = n * NB;
-    k_dense_mv<T><<<gm_blocks(nd, 128), 128, 0, st>>>(nd, (const T*)lv.dense_inv, b, x);
+    k_dense_mv<T><<<gm_blocks(nd, 128), 128, 0, st>>>(nd, (const double*)lv.dense_inv, b, x);
     GM_LAUNCH_CHECK();
     return 0;
   }
   int sweeps = coarsest ? cy.coarse_sweeps : cy.nu_pre;
   k_jac_first<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, cy.omega, x);
   for (int k = 1; k < sweeps; ++k) {
-    launch_spmv<NB, T, T, T>(n, lv.row_ptr, lv.col_idx, vals, x, r, nullptr, nullptr, nullptr, st);
+    level_spmv<NB, T>(lv, l, x, r, st);
     k_jac_update<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, r, cy.omega, x);
   }
   GM_LAUNCH_CHECK();
@@ -1210,7 +1342,7 @@ static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
   const gm_mg_level& cv = cy.L[l + 1];
   const int gamma = lv.gamma > 1 ? lv.gamma : 1;  // 1: V-cycle, 2: W-cycle at this level
   for (int g = 0; g < gamma; ++g) {
-    launch_spmv<NB, T, T, T>(n, lv.row_ptr, lv.col_idx, vals, x, r, nullptr, nullptr, nullptr, st);
+    level_spmv<NB, T>(lv, l, x, r, st);
     long nv = (long)n * NB;
     k_sub_inplace<T><<<dot_blocks(nv), RED_T, 0, st>>>(nv, b, r);
     k_restrict<DIM, NB, T><<<gm_blocks(cv.n_nodes, 256), 256, 0, st>>>(dims_of(lv), dims_of(cv), cv.n_nodes,
@@ -1221,7 +1353,7 @@ static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
     k_prolong_add<DIM, NB, T><<<rb, 256, 0, st>>>(dims_of(lv), dims_of(cv), n, lv.freem, (const T*)cv.x, x);
   }
   for (int k = 0; k < cy.nu_post; ++k) {
-    launch_spmv<NB, T, T, T>(n, lv.row_ptr, lv.col_idx, vals, x, r, nullptr, nullptr, nullptr, st);
+    level_spmv<NB, T>(lv, l, x, r, st);
     k_jac_update<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, r, cy.omega, x);
   }
   GM_LAUNCH_CHECK();
@@ -1238,23 +1370,46 @@ static int setup(int n_levels, gm_mg_level* L, void* ap_work, hipStream_t st) {
     const int TT = (DIM == 2) ? 16 : 64;
     const int W = (DIM == 2) ? 25 : 125;
     long nap = (long)f.n_nodes * TT;
-    k_ap<DIM, NB, T><<<gm_blocks(nap, 256), 256, 0, st>>>(df, dc, f.n_nodes, f.row_ptr, f.row_mask,
-                                                           (const T*)f.vals, f.freem, (T*)ap_work);
+    // Galerkin products in FP64 (the AP workspace and the coarse operators are FP64 in
+    // both precisions: FP32 rounding of the cancelling sums corrupts deep coarse levels)
+    if (l == 0)
+      k_ap<DIM, NB, T><<<gm_blocks(nap, 256), 256, 0, st>>>(df, dc, f.n_nodes, f.row_ptr, f.row_mask,
+                                                             (const T*)f.vals, f.freem, (double*)ap_work);
+    else
+      k_ap<DIM, NB, double><<<gm_blocks(nap, 256), 256, 0, st>>>(df, dc, f.n_nodes, f.row_ptr, f.row_mask,
+                                                                  (const double*)f.vals, f.freem, (double*)ap_work);
     long nrap = (long)c.n_nodes * W;
-    k_rap<DIM, NB, T><<<gm_blocks(nrap, 256), 256, 0, st>>>(df, dc, c.n_nodes, c.row_ptr, c.row_mask, (T*)c.vals,
-                                                             f.freem, c.freem, (const T*)ap_work);
+    k_rap<DIM, NB><<<gm_blocks(nrap, 256), 256, 0, st>>>(df, dc, c.n_nodes, c.row_ptr, c.row_mask,
+                                                          (double*)c.vals, f.freem, c.freem, (const double*)ap_work);
     GM_LAUNCH_CHECK();
   }
-  for (int l = 0; l < n_levels; ++l)
-    k_diag_inv<NB, T><<<gm_blocks(L[l].n_nodes, 256), 256, 0, st>>>(L[l].n_nodes, L[l].row_ptr, L[l].col_idx,
-                                                                     (const T*)L[l].vals, (T*)L[l].dinv);
+  for (int l = 0; l < n_levels; ++l) {
+    int nl = L[l].n_nodes;
+    if (l == 0)
+      k_diag_inv<NB, T, T><<<gm_blocks(nl, 256), 256, 0, st>>>(nl, L[l].row_ptr, L[l].col_idx, (const T*)L[l].vals,
+                                                                (T*)L[l].dinv);
+    else
+      k_diag_inv<NB, double, T><<<gm_blocks(nl, 256), 256, 0, st>>>(nl, L[l].row_ptr, L[l].col_idx,
+                                                                     (const double*)L[l].vals, (T*)L[l].dinv);
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    for (int l = 0; l < n_levels; ++l) {
+      if (!L[l].vals_lo) continue;
+      if (l == 0)
+        launch_lo_convert<NB, float>(L[l].n_nodes, L[l].row_ptr, (const float*)L[l].vals, (gm_half*)L[l].vals_lo,
+                                     L[l].row_scale, st);
+      else
+        launch_lo_convert<NB, double>(L[l].n_nodes, L[l].row_ptr, (const double*)L[l].vals,
+                                      (gm_half*)L[l].vals_lo, L[l].row_scale, st);
+    }
+  }
   GM_LAUNCH_CHECK();
   return 0;
 }
 
 }  // namespace mg
 
-static int mg_check(int dim, int nb, int n_levels, const gm_mg_level* levels) {
+static int mg_check(int dim, int nb, int n_levels, const gm_mg_level* levels, int fp32) {
   GM_REQUIRE(levels && n_levels >= 1 && n_levels <= 16, "multigrid: bad level list");
   GM_REQUIRE(dim == 2 || dim == 3, "multigrid: dim must be 2 or 3");
   GM_REQUIRE(nb >= 1 && nb <= 4, "multigrid: nb must be 1..4");
@@ -1264,6 +1419,8 @@ static int mg_check(int dim, int nb, int n_levels, const gm_mg_level* levels) {
                "multigrid: level array missing");
     long expect = (long)(v.nx + 1) * (v.ny + 1) * (dim == 3 ? (v.nz + 1) : 1);
     GM_REQUIRE(expect == v.n_nodes, "multigrid: n_nodes does not match the level's grid");
+    GM_REQUIRE(!v.vals_lo || (fp32 && v.row_scale), "multigrid: an FP16 level copy needs an FP32 hierarchy "
+               "and row scales");
     if (l > 0) {
       const gm_mg_level& f = levels[l - 1];
       GM_REQUIRE(v.x && v.b, "multigrid: coarse level vectors missing");
@@ -1291,7 +1448,7 @@ static int mg_check(int dim, int nb, int n_levels, const gm_mg_level* levels) {
 
 extern "C" int gm_mg_setup(int dim, int nb, int fp32, int n_levels, gm_mg_level* levels, void* ap_work,
                            hipStream_t s) {
-  int rc = mg_check(dim, nb, n_levels, levels);
+  int rc = mg_check(dim, nb, n_levels, levels, fp32);
   if (rc) return rc;
   GM_REQUIRE(n_levels == 1 || ap_work, "gm_mg_setup: missing AP workspace");
 #define GM_SETUP(D, B)                                                                      \
@@ -1304,7 +1461,7 @@ extern "C" int gm_mg_setup(int dim, int nb, int fp32, int n_levels, gm_mg_level*
 
 extern "C" int gm_mg_vcycle(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre,
                             int nu_post, int coarse_sweeps, double omega, const void* b, void* x, hipStream_t s) {
-  int rc = mg_check(dim, nb, n_levels, levels);
+  int rc = mg_check(dim, nb, n_levels, levels, fp32);
   if (rc) return rc;
   GM_REQUIRE(b && x && nu_pre >= 1 && nu_post >= 0 && coarse_sweeps >= 1, "gm_mg_vcycle: bad argument");
   mg::Cycle cy{dim, n_levels, nu_pre, nu_post, coarse_sweeps, omega, levels};
@@ -1337,7 +1494,7 @@ extern "C" int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_
                            int nu_post, int coarse_sweeps, double omega, const void* b, void* x, void* V, void* Z,
                            void* z, void* w, double* part, double* dh, int m, double tol, int maxiter, int* iters,
                            double* relres, hipStream_t s) {
-  int rc = mg_check(dim, nb, n_levels, levels);
+  int rc = mg_check(dim, nb, n_levels, levels, fp32);
   if (rc) return rc;
   GM_REQUIRE(iters && relres && m >= 1 && m <= 256 && b && x && V && Z && z && w, "gm_gmres_mg: bad argument");
   GM_REQUIRE(nu_pre >= 1 && nu_post >= 0 && coarse_sweeps >= 1, "gm_gmres_mg: bad smoother counts");
@@ -1504,7 +1661,7 @@ extern "C" int gm_up_setup(int dim, int nb, int fp32, int n_rows, int nnzb, cons
   GM_REQUIRE(nb == 3 || nb == 4, "gm_up_setup: nb must be 3 (2D u-p) or 4 (3D u-p)");
   GM_REQUIRE(dim == nb - 1, "gm_up_setup: nb must be dim + 1");
   GM_REQUIRE(row_ptr && col_idx && vals && freem && sinv && ulevels, "gm_up_setup: null argument");
-  int rc = mg_check(dim, nb - 1, n_levels, ulevels);
+  int rc = mg_check(dim, nb - 1, n_levels, ulevels, fp32);
   if (rc) return rc;
   GM_REQUIRE(ulevels[0].n_nodes == n_rows && ulevels[0].row_ptr == row_ptr && ulevels[0].col_idx == col_idx,
              "gm_up_setup: u-level 0 must share the system's pattern");
@@ -1539,7 +1696,7 @@ extern "C" int gm_gmres_up(int dim, int nb, int fp32, int n_rows, const int* row
                            int maxiter, int* iters, double* relres, hipStream_t s) {
   GM_REQUIRE(nb == 3 || nb == 4, "gm_gmres_up: nb must be 3 or 4");
   GM_REQUIRE(dim == nb - 1, "gm_gmres_up: nb must be dim + 1");
-  int rc = mg_check(dim, nb - 1, n_levels, ulevels);
+  int rc = mg_check(dim, nb - 1, n_levels, ulevels, fp32);
   if (rc) return rc;
   GM_REQUIRE(iters && relres && m >= 1 && m <= 256 && b && x && V && Z && z && w && vu && zu && sinv,
              "gm_gmres_up: bad argument");

@@ -78,7 +78,7 @@ typedef struct gm_stage_args {
 } gm_stage_args;
 
 /* Krylov workspace (device pointers).  Vectors hold n_rows*nb entries of the
-   solve precision; dinv holds n_rows*nb*nb; part >= 2*max(n_rows/32+1, 1024)
+   solve precision; dinv holds n_rows*nb*nb; part >= 2*max(n_rows/16+1, 1024)
    doubles; sc >= 16 doubles. */
 typedef struct gm_krylov_ws {
   void* r; void* rhat; void* p; void* v; void* s; void* t; void* y; void* z;  /* n_rows*nb each, TV */
@@ -140,12 +140,18 @@ typedef struct gm_mg_level {
   const int* row_ptr;
   const int* col_idx;
   const uint64_t* row_mask;
-  void* vals;           /* nnzb * nb * nb */
+  void* vals;           /* nnzb * nb * nb: level 0 in the solve precision, coarser levels FP64 */
   void* dinv;           /* n_nodes * nb * nb */
   uint8_t* freem;       /* n_nodes * nb: 1 = free dof */
   void* x; void* b; void* r;  /* n_nodes * nb work vectors (level 0 x/b unused) */
-  const void* dense_inv;      /* coarsest level only, may be null: (n_nodes*nb)^2 inverse, row-major */
+  const void* dense_inv;      /* coarsest level only, may be null: (n_nodes*nb)^2 FP64 inverse, row-major */
   int gamma;                  /* coarse-grid corrections from this level per cycle (<= 1: V, 2: W) */
+  /* optional, FP32 hierarchies only: an FP16 copy of vals, each (row, column dof)
+     sub-row of a block row scaled by its largest magnitude,
+     vals[k][r][c] ~ vals_lo[k][r][c] * row_scale[(row * nb + r) * nb + c], filled by
+     gm_mg_setup; when set, the V-cycle's smoothing and residual products on this level read it */
+  void* vals_lo;              /* nnzb * nb * nb halves, or null */
+  float* row_scale;           /* n_nodes * nb * nb */
 } gm_mg_level;
 
 #ifdef __cplusplus

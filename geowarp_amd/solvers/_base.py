@@ -13,9 +13,15 @@ Extensions over the reference (all optional, defaults reproduce it):
                                   FP32 Krylov, FP64 residual/refinement)
   iteration_dict['linear_tol'], ['linear_maxiter'], ['ir_steps'], ['forcing']
                                   ('fixed' | 'ew': Eisenstat-Walker inner tolerance),
+  ['forcing_max']                 cap of the 'ew' inner tolerance (1e-2; 1e-4 for J2 and
+                                  Drucker-Prager), ['line_search'], ['autotune'],
   ['krylov'] ('mg' default: multigrid-preconditioned flexible GMRES |
   'gmres': block-Jacobi GMRES | 'bicgstab'), ['krylov_restart'],
   ['mg_nu'], ['mg_omega'], ['mg_coarse_sweeps'], ['mg_ublock']  V-cycle options
+  ['mg_lowprec']                  mixed form: FP16 (row-scaled) smoother matrices (default True)
+  ['mixed_assembly']              mixed form: 'fp32' tangent and Jacobian (default) or 'fp64'
+                                  (FP64 assembly, FP32 copy for the Krylov solve, refinement
+                                  residual against the FP64 Jacobian)
   iteration_dict['verbose']       print residual norms like the reference
   material_dict['H']              J2 linear isotropic hardening modulus
   material_dict['friction_angle'], ['dilation_angle'], ['cohesion'],
@@ -108,15 +114,21 @@ class _QuasiStaticBase:
         self.line_search_steps = []
         # multigrid V-cycle options (krylov='mg'): smoothing sweeps, damping, coarsest-level sweeps
         mg_opts = {k: it[k2] for k, k2 in (('nu_pre', 'mg_nu'), ('nu_post', 'mg_nu'), ('omega', 'mg_omega'),
-                                           ('coarse_sweeps', 'mg_coarse_sweeps'), ('ublock', 'mg_ublock'))
+                                           ('coarse_sweeps', 'mg_coarse_sweeps'), ('ublock', 'mg_ublock'),
+                                           ('lowprec', 'mg_lowprec'))
                    if k2 in it}
         self.krylov_restart = it.get('krylov_restart', 250 if self.krylov_method == 'mg' else 40)
         # inner tolerance: 'fixed' (linear_tol) or 'ew' (Eisenstat-Walker choice 2)
         self.forcing = it.get('forcing', 'fixed' if self.precision == 'fp64' else 'ew')
-        self.eta_max = it.get('forcing_max', 1e-2)  # cap of the Eisenstat-Walker inner tolerance
         self.verbose = it.get('verbose', False)
         m = material_dict
         self.material_name = m['material_name']
+        # cap of the Eisenstat-Walker inner tolerance.  Materials with a return mapping get a
+        # tighter cap: a loose Newton correction moves particles across the yield surface, and
+        # the iteration then stalls on the changed active set (a 256 x 1024-cell J2 bar stalls
+        # at relative residual 3e-2 with a 1e-2 cap in FP64 as in FP32, and converges
+        # quadratically with 1e-4)
+        self.eta_max = it.get('forcing_max', 1e-4 if self.material_name in ('J2', 'Drucker-Prager') else 1e-2)
         if self.material_name not in MAT_KIND:
             raise ValueError("Unimplemented material model: %s" % self.material_name)
         self.youngs_modulus = m['youngs_modulus']
@@ -134,6 +146,7 @@ class _QuasiStaticBase:
         spacing = (self.dx, self.dx, 0.0) if self.dim == 2 else (self.dx, self.dy, self.dz)
         n_grid = (self.n_grid_x, self.n_grid_y) if self.dim == 2 else (self.n_grid_x, self.n_grid_y, self.n_grid_z)
         self.engine = Engine(self.dim, self.coupled, n_grid, spacing, device=self.device, precision=self.precision)
+        self.engine.mixed_assembly = it.get('mixed_assembly', 'fp32')
         self.engine.mg_options = mg_opts
         dev = self.device
         self.dofStruct = DofStruct(self.n_matrix_size, dev)
@@ -302,7 +315,7 @@ class _QuasiStaticBase:
         reach the Newton tolerance."""
         gamma, alpha = 0.9, 2.0
         if nr_prev is None or eta_prev is None:
-            eta = 1e-2
+            eta = min(1e-2, self.eta_max)
         else:
             eta = gamma * (nr / nr_prev) ** alpha
             sg = gamma * eta_prev ** alpha

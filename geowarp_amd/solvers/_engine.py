@@ -93,12 +93,19 @@ class Engine:
         self.jac_version = 0
         self.mg_options = {}
         self.mixed_inner_maxiter = 400  # FP32 Krylov iterations per refinement sweep
+        # mixed form: 'fp32' assembles the tangent and the Jacobian in FP32; 'fp64' assembles
+        # them in FP64, solves with an FP32 copy and refines against the FP64 Jacobian
+        self.mixed_assembly = 'fp32'
 
     # ------------------------------------------------------------------ buffers
     def _alloc_vals(self):
         jdt = torch.float32 if self.precision == 'mixed' else torch.float64
         self.vals = torch.zeros(self.cap * self.nb * self.nb, dtype=jdt, device=self.dev)
+        self.vals64 = None  # FP64 Jacobian of the mixed form with FP64 assembly (allocated on first use)
         self.mvals = None
+
+    def _tangent_fp32(self):
+        return self.precision == 'mixed' and getattr(self, 'mixed_assembly', 'fp32') == 'fp32'
 
     def _grow(self, nnzb):
         self.cap = int(nnzb * 1.15) + 16
@@ -117,7 +124,7 @@ class Engine:
         self.x_s = torch.zeros(max(n, 1) * self.dim, dtype=torch.float64, device=dev)
         self.lp_s = torch.zeros(max(n, 1) * self.dim, dtype=torch.float64, device=dev)
         self.Y = torch.zeros(max(n, 1) * self.NY, dtype=torch.float64, device=dev)
-        ddt = torch.float32 if self.precision == 'mixed' else torch.float64
+        ddt = torch.float32 if self._tangent_fp32() else torch.float64
         self.D = torch.zeros(max(n, 1) * self.NY * self.NZ, dtype=ddt, device=dev)
         self.pen_force = torch.zeros(max(n, 1), dtype=torch.float64, device=dev)
 
@@ -128,7 +135,7 @@ class Engine:
             n = self.n_nodes * nb
             vecs = [torch.zeros(n, dtype=dt, device=self.dev) for _ in range(8)]
             dinv = torch.zeros(self.n_nodes * nb * nb, dtype=dt, device=self.dev)
-            part = torch.zeros(2 * max(self.n_nodes // 32 + 2, 2048), dtype=torch.float64, device=self.dev)
+            part = torch.zeros(2 * max(self.n_nodes // 16 + 2, 2048), dtype=torch.float64, device=self.dev)
             sc = torch.zeros(16, dtype=torch.float64, device=self.dev)
             ws = L.gm_krylov_ws(*[v.data_ptr() for v in vecs], dinv.data_ptr(), part.data_ptr(), sc.data_ptr())
             self._ws[key] = (ws, vecs, dinv, part, sc)
@@ -231,8 +238,11 @@ class Engine:
                                               _stream()), 'gm_set_nodal_pressure')
 
     def stage(self, args, want_tangent):
+        t32 = self._tangent_fp32()
+        if self.D.dtype != (torch.float32 if t32 else torch.float64):
+            self.D = torch.zeros(self.D.numel(), dtype=torch.float32 if t32 else torch.float64, device=self.dev)
         args.want_tangent = int(want_tangent)
-        args.tangent_fp32 = int(self.precision == 'mixed')
+        args.tangent_fp32 = int(t32)
         args.sorted = self.sorted.data_ptr()
         args.Y = self.Y.data_ptr()
         args.D = self.D.data_ptr()
@@ -274,6 +284,17 @@ class Engine:
     def jacobian(self, x, lp):
         fp32 = self.precision == 'mixed'
         self.jac_version += 1
+        if fp32 and not self._tangent_fp32():
+            # FP64 assembly, FP32 copy for the Krylov solve
+            if self.vals64 is None or self.vals64.numel() != self.vals.numel():
+                self.vals64 = torch.zeros(self.vals.numel(), dtype=torch.float64, device=self.dev)
+            L.check(L.lib().gm_node_jacobian(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins),
+                                             L.ctypes.byref(self.pattern), L.ptr(self.D), 0, L.ptr(self.freem),
+                                             L.ptr(self.vals64), 0, int(self.coupled), _stream()),
+                    'gm_node_jacobian')
+            L.check(L.lib().gm_d2f(self.nnzb * self.nb * self.nb, L.ptr(self.vals64), L.ptr(self.vals), _stream()),
+                    'gm_d2f')
+            return
         L.check(L.lib().gm_node_jacobian(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins),
                                          L.ctypes.byref(self.pattern), L.ptr(self.D), int(fp32), L.ptr(self.freem),
                                          L.ptr(self.vals), int(fp32), int(self.coupled), _stream()),
@@ -377,9 +398,11 @@ class Engine:
                 if rel_in <= tol:
                     break  # the FP32 solve met its tolerance: no further sweep
                 # FP64 refinement residual b - K delta; stop once it meets the tolerance
-                L.check(lib.gm_residual_refine(self.nb, 1, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),
-                                               L.ptr(self.vals), L.ptr(b), L.ptr(self.delta), L.ptr(self._ax64),
-                                               L.ptr(self._res64), _stream()), 'gm_residual_refine')
+                k64 = not self._tangent_fp32() and not equilibrate  # refine against the FP64 Jacobian
+                L.check(lib.gm_residual_refine(self.nb, 0 if k64 else 1, self.n_nodes, L.ptr(self.row_ptr),
+                                               L.ptr(self.col_idx), L.ptr(self.vals64 if k64 else self.vals),
+                                               L.ptr(b), L.ptr(self.delta), L.ptr(self._ax64), L.ptr(self._res64),
+                                               _stream()), 'gm_residual_refine')
                 src = self._res64
                 if bnorm is None:
                     bnorm = float(torch.linalg.norm(b))

@@ -56,9 +56,11 @@ class Multigrid:
     block-triangular u-p preconditioner (gm_up_setup / gm_gmres_up)."""
 
     def __init__(self, eng, fp32, max_levels=8, min_cells=2, nu_pre=3, nu_post=3, coarse_sweeps=20, omega=0.4,
-                 max_dense=3000, ublock=False, gamma=1):
+                 max_dense=3000, ublock=False, gamma=1, lowprec=None):
         self.eng = eng
         self.fp32 = bool(fp32)
+        # FP16 copies of the level matrices for the smoother (FP32 hierarchies; default on)
+        self.lowprec = self.fp32 and (True if lowprec is None else bool(lowprec))
         self.dim = eng.dim
         self.ublock = bool(ublock) and eng.coupled
         self.nb = eng.nb - 1 if self.ublock else eng.nb
@@ -88,7 +90,7 @@ class Multigrid:
                 rp, ci, mk = window_pattern(self.dim, (nx, ny, nz))
                 bufs.update(rp=torch.from_numpy(rp).to(dev), ci=torch.from_numpy(ci).to(dev),
                             mask=torch.from_numpy(mk.reshape(-1).copy()).to(dev),
-                            vals=torch.zeros(len(ci) * nb2, dtype=dt, device=dev),
+                            vals=torch.zeros(len(ci) * nb2, dtype=torch.float64, device=dev),
                             freem=torch.zeros(n * self.nb, dtype=torch.uint8, device=dev),
                             x=torch.zeros(n * self.nb, dtype=dt, device=dev),
                             b=torch.zeros(n * self.nb, dtype=dt, device=dev))
@@ -97,10 +99,15 @@ class Multigrid:
                                                        bufs['mask'].data_ptr())
                 lv.vals, lv.freem = bufs['vals'].data_ptr(), bufs['freem'].data_ptr()
                 lv.x, lv.b = bufs['x'].data_ptr(), bufs['b'].data_ptr()
+                if self.lowprec:
+                    bufs.update(lo=torch.zeros(len(ci) * nb2, dtype=torch.float16, device=dev),
+                                scale=torch.zeros(n * nb2, dtype=torch.float32, device=dev))
+                    lv.vals_lo, lv.row_scale = bufs['lo'].data_ptr(), bufs['scale'].data_ptr()
             lv.dinv, lv.r = bufs['dinv'].data_ptr(), bufs['r'].data_ptr()
             self._keep.append(bufs)
         n0 = self.levels[0].n_nodes
-        self.ap = torch.zeros(n0 * (4 ** self.dim) * nb2, dtype=dt, device=dev) if len(dims) > 1 else None
+        # Galerkin workspace, coarse operators and the coarsest inverse are FP64 in both precisions
+        self.ap = torch.zeros(n0 * (4 ** self.dim) * nb2, dtype=torch.float64, device=dev) if len(dims) > 1 else None
         self.dt = dt
         if self.ublock:
             # level 0 holds the extracted displacement block; the Schur diagonal and split vectors
@@ -113,7 +120,7 @@ class Multigrid:
         # the coarsest level is solved exactly through a dense inverse when it is small enough
         nc = self.levels[len(dims) - 1].n_nodes * self.nb
         self.dense = len(dims) > 1 and nc <= max_dense
-        self.cinv = torch.zeros(nc * nc, dtype=dt, device=dev) if self.dense else None
+        self.cinv = torch.zeros(nc * nc, dtype=torch.float64, device=dev) if self.dense else None
         self.levels[len(dims) - 1].dense_inv = self.cinv.data_ptr() if self.dense else None
 
     @property
@@ -127,6 +134,14 @@ class Multigrid:
         lv.nnzb = eng.nnzb
         lv.row_ptr, lv.col_idx, lv.row_mask = eng.row_ptr.data_ptr(), eng.col_idx.data_ptr(), \
             eng.row_mask.data_ptr()
+        if self.lowprec:
+            k0 = self._keep[0]
+            need = max(eng.nnzb, 1) * self.nb * self.nb
+            if k0.get('lo') is None or k0['lo'].numel() < need:
+                k0['lo'] = torch.zeros(int(need * 1.15) + 16, dtype=torch.float16, device=eng.dev)
+            if k0.get('scale') is None:
+                k0['scale'] = torch.zeros(lv.n_nodes * self.nb * self.nb, dtype=torch.float32, device=eng.dev)
+            lv.vals_lo, lv.row_scale = k0['lo'].data_ptr(), k0['scale'].data_ptr()
         if self.ublock:
             k0 = self._keep[0]
             need = max(eng.nnzb, 1) * self.nb * self.nb
@@ -154,7 +169,7 @@ class Multigrid:
             for r in range(nb):
                 for c in range(nb):
                     A[brow * nb + r, ci * nb + c] += v[:, r, c]
-            self.cinv.copy_(torch.from_numpy(_robust_inverse(A).reshape(-1)).to(self.cinv.dtype))
+            self.cinv.copy_(torch.from_numpy(_robust_inverse(A).reshape(-1)))
 
     def gmres(self, b, x, V, z, w, part, dh, m, tol, maxiter, stream):
         eng = self.eng

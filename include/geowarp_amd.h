@@ -224,8 +224,11 @@ int gm_bsr_transpose(int nb, int fp32, int n_rows, const int* row_ptr, const int
 /* Build levels 1..n_levels-1 from level 0 (whose pattern, values and free mask
    are the Jacobian's) and every level's block-diagonal inverse.  Coarse
    patterns (row_ptr, col_idx, row_mask: full 5^dim windows clipped to the
-   grid) and all buffers are the caller's.  ap_work: level-0 n_nodes * 4^dim *
-   nb * nb values of the solve precision. */
+   grid) and all buffers are the caller's.  Level 0 holds the solve precision;
+   the Galerkin products, the coarse operators (levels 1.. vals) and a coarsest
+   dense inverse are FP64 in both precisions (the smoothers of an FP32 hierarchy
+   run in FP32, or from the FP16 copies when vals_lo is set).  ap_work: level-0
+   n_nodes * 4^dim * nb * nb doubles. */
 int gm_mg_setup(int dim, int nb, int fp32, int n_levels, gm_mg_level* levels, void* ap_work, hipStream_t stream);
 
 /* x = M^-1 b for one V-cycle (nu_pre / nu_post damped block-Jacobi sweeps with

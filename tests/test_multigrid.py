@@ -9,6 +9,13 @@
   with an independent SciPy residual, in fewer iterations than block Jacobi.
 * Whole load steps with the multigrid solve reproduce the oracle fixtures
   (1e-8 relative, as the other parity tests).
+* FP32 hierarchies: the scaled FP16 copy of each level matrix (the smoother's
+  operator) reproduces the FP32 matrix to FP16 rounding relative to its scale,
+  the scale of each (row, column dof) sub-row is its largest magnitude, one V-cycle with the
+  copy stays within 1e-2 of the V-cycle without it, and MG-GMRES with it
+  reaches its tolerance (SciPy residual of the FP32 matrix).
+* Block-Jacobi BiCGStab on a 3D u-p system (16 lanes per block row, fused dot
+  products) reaches its tolerance (SciPy residual).
 """
 import itertools
 import os
@@ -55,10 +62,10 @@ def _prolongation(dim, fine, coarse, nb, freem_f):
     return sp.csr_matrix((vals, (rows, cols)), shape=(int(np.prod(nf)) * nb, int(np.prod(nc)) * nb))
 
 
-def _first_system(name, krylov='mg'):
+def _first_system(name, krylov='mg', precision='fp64'):
     import geowarp_amd.solvers as gs
     cfg = H.MINI[name]()
-    cfg['iteration_dict'] = dict(cfg['iteration_dict'], krylov=krylov)
+    cfg['iteration_dict'] = dict(cfg['iteration_dict'], krylov=krylov, precision=precision)
     s = H.make(gs, cfg)
     total = cfg['total_steps']
     s.prepare_step(0, total)
@@ -142,3 +149,77 @@ def test_load_steps_with_multigrid_match_oracle(name):
             else:
                 e = H.rel_err(f[key], g[gk])
             assert e < 1e-8, (name, k, key, e)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', ['indent_nh', 'terzaghi', 'bar_j2'])
+def test_fp16_smoother_copy(name):
+    from geowarp_amd import _lib as L
+    from geowarp_amd.solvers._engine import _stream
+    from geowarp_amd.solvers._multigrid import Multigrid
+    s, eng = _first_system(name, precision='mixed')
+    assert eng.vals.dtype == torch.float32
+    opts = {k: v for k, v in eng.mg_options.items() if k != 'lowprec'}
+    mg_lo = Multigrid(eng, True, lowprec=True, **opts)
+    mg_hi = Multigrid(eng, True, lowprec=False, **opts)
+    st = _stream()
+    mg_lo.setup(st)
+    mg_hi.setup(st)
+    torch.cuda.synchronize()
+    nb = mg_lo.nb
+    for l in range(mg_lo.n_levels):
+        k = mg_lo._keep[l]
+        lv = mg_lo.levels[l]
+        if l == 0:
+            rp = eng.row_ptr.long()
+            A_src = k['vals'] if mg_lo.ublock else eng.vals
+        else:
+            rp = k['rp'].long()
+            A_src = k['vals']
+        nnzb = int(rp[-1])
+        A = A_src[:nnzb * nb * nb].double().reshape(-1, nb, nb)
+        lo = k['lo'][:nnzb * nb * nb].double().reshape(-1, nb, nb)
+        sc = k['scale'][:lv.n_nodes * nb * nb].double().reshape(-1, nb, nb)
+        brow = torch.repeat_interleave(torch.arange(lv.n_nodes, device=A.device), rp[1:] - rp[:-1])
+        rowmax = torch.zeros(lv.n_nodes, nb * nb, dtype=torch.float64, device=A.device)
+        rowmax.scatter_reduce_(0, brow[:, None].expand(-1, nb * nb), A.abs().reshape(-1, nb * nb), reduce='amax')
+        rowmax = rowmax.reshape(-1, nb, nb)
+        expect = torch.where(rowmax > 0, rowmax, torch.ones_like(rowmax))
+        assert torch.equal(sc, expect.float().double()), l
+        err = float(((lo * sc[brow] - A).abs() / sc[brow]).max())
+        print(name, 'level', l, 'nnzb', nnzb, 'max scaled fp16 error', err)
+        assert err <= 2.0 ** -11
+    if not mg_lo.ublock:
+        n = eng.n_nodes * nb
+        b = torch.randn(n, dtype=torch.float32, device=eng.dev) * eng.freem.float()
+        xs = []
+        for mg in (mg_lo, mg_hi):
+            x = torch.zeros(n, dtype=torch.float32, device=eng.dev)
+            L.check(L.lib().gm_mg_vcycle(eng.dim, nb, 1, mg.n_levels, mg.levels, mg.nu_pre, mg.nu_post,
+                                         mg.coarse_sweeps, mg.omega, L.ptr(b), L.ptr(x), st), 'gm_mg_vcycle')
+            xs.append(x.double())
+        torch.cuda.synchronize()
+        d = float((xs[0] - xs[1]).norm() / xs[1].norm())
+        print(name, 'V-cycle with / without the FP16 copy: rel diff', d)
+        assert d < 1e-2
+    # the engine's mixed-form hierarchy uses the copy by default
+    assert eng.multigrid(True)[0].lowprec
+    A = _bsr_interleaved(eng.row_ptr, eng.col_idx, eng.vals, eng.n_nodes, eng.nb)
+    b32 = eng.rhs.float()
+    x = torch.zeros_like(b32)
+    its, rel = eng.krylov(b32, x, True, 1e-6, 2000, method='mg', restart=250)
+    r = np.linalg.norm(A @ x.double().cpu().numpy() - b32.double().cpu().numpy()) / float(b32.double().norm())
+    print(name, 'mixed MG-GMRES with FP16 smoother: its', its, 'relres', rel, 'scipy', r)
+    assert r < 5e-6
+
+
+@pytest.mark.gpu
+def test_bicgstab_on_3d_up_system():
+    s, eng = _first_system('indent_nh', krylov='bicgstab')
+    A = _bsr_interleaved(eng.row_ptr, eng.col_idx, eng.vals, eng.n_nodes, eng.nb)
+    b = eng.rhs.cpu().numpy()
+    x = torch.zeros_like(eng.rhs)
+    its, rel = eng.krylov(eng.rhs, x, False, 1e-10, 20000, method='bicgstab')
+    r = np.linalg.norm(A @ x.cpu().numpy() - b) / np.linalg.norm(b)
+    print('bicgstab its', its, 'relres', rel, 'scipy relres', r)
+    assert rel <= 1e-10 and r < 1e-8
