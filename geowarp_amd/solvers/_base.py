@@ -297,6 +297,7 @@ class _QuasiStaticBase:
         eng = self.engine
         base = u_new.clone()
         a = 1.0
+        finite = False
         for _ in range(max_halvings + 1):
             u_new.copy_(base)
             eng.axpy(a, eng.delta, u_new)
@@ -304,9 +305,14 @@ class _QuasiStaticBase:
             eng.residual(self.x_particles, self.GIMP_lp)
             eng.norm2(eng.rhs, 0)
             trial = math.sqrt(max(eng.scal[0].item(), 0.0))
-            if math.isfinite(trial) and trial <= (1.0 - 1e-4 * a) * nr:
+            finite = math.isfinite(trial)
+            if finite and trial <= (1.0 - 1e-4 * a) * nr:
                 break
-            a *= 0.5
+            if _ < max_halvings:
+                a *= 0.5
+        if not finite:
+            u_new.copy_(base)
+            raise L.GMError('line search: no finite residual along the Newton correction (down to step %g)' % a)
         self.line_search_steps.append(a)
 
     def _ew_forcing(self, nr, nr_prev, eta_prev, r0):

@@ -92,7 +92,7 @@ class Engine:
         self.stats = dict(krylov_iters=[], krylov_relres=[])
         self.jac_version = 0
         self.mg_options = {}
-        self.mixed_inner_maxiter = 400  # FP32 Krylov iterations per refinement sweep
+        self.mixed_inner_maxiter = None  # FP32 Krylov iterations per refinement sweep (None: maxiter)
         # mixed form: 'fp32' assembles the tangent and the Jacobian in FP32; 'fp64' assembles
         # them in FP64, solves with an FP32 copy and refines against the FP64 Jacobian
         self.mixed_assembly = 'fp32'
@@ -324,6 +324,11 @@ class Engine:
             if method != 'mg':
                 raise
             its, rel = maxiter, float('inf')
+        if not bool(torch.isfinite(x).all()):
+            x.zero_()
+            if method != 'mg':
+                raise L.GMError('%s: the Krylov solution is not finite' % method)
+            rel = float('inf')
         if method == 'mg' and not (rel <= 1.0):
             self.stats.setdefault('fallbacks', 0)
             self.stats['fallbacks'] += 1
@@ -389,7 +394,7 @@ class Engine:
         self.delta.zero_()
         total = 0
         bnorm = None
-        inner_max = min(maxiter, self.mixed_inner_maxiter)
+        inner_max = maxiter if self.mixed_inner_maxiter is None else min(maxiter, self.mixed_inner_maxiter)
         rel_in = float('inf')
         for k in range(ir_steps):
             if k == 0:
