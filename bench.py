@@ -177,7 +177,8 @@ def adjoint_bench(steps, total=30, every=5, n_specimens=64, krylov='mg'):
                          % (n_specimens, s.n_particles, int(W.min()), int(W.max()), steps - 1, total, every),
                 seconds_forward=round(t1 - t0, 3), seconds_adjoint=round(t2 - t1, 3),
                 grads_per_s=round(1.0 / (t2 - t0), 4), gradient_components=2 * n_specimens,
-                forward_steps=adj.stats['forward_steps'], adjoint_steps=adj.stats['adjoint_steps'],
+                forward_steps=adj.stats['forward_steps'], replayed_steps=adj.stats['replayed_steps'],
+                adjoint_steps=adj.stats['adjoint_steps'],
                 transposed_solve_iters=dict(min=min(kits), max=max(kits), mean=round(sum(kits) / len(kits), 1)),
                 unconverged_forward_steps=unconverged,
                 grad_norm=dict(phi=float(phi_t.grad.norm()), coh=float(coh_t.grad.norm())),

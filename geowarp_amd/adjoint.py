@@ -27,7 +27,12 @@ state cotangent Sbar_{k+1} carried back from step k+1:
 The per-particle VJPs are forward-mode dual-number evaluations of the particle
 map, one direction per input (gm_local_vjp_2d).  Load steps are recomputed
 from checkpoints taken every ``checkpoint_every`` steps, so memory is one
-checkpoint per segment plus one segment of states.
+particle-state checkpoint per segment plus one segment of states.  The forward
+run also keeps each step's converged nodal increment (store_increments, default
+on: n_nodes x 2 doubles per step), so a recomputation replays the step (P2G,
+G2P) without its Newton solve; the replayed states differ from the forward
+ones only by the last Newton correction (the forward commits the stage of the
+iterate before it).
 
 Discrete choices (active nodes, particle supports, the platen rows) are held
 fixed: the gradient is that of the piecewise-smooth branch the forward run
@@ -59,7 +64,7 @@ class LoadPathAdjoint:
     """
 
     def __init__(self, solver, total_steps, n_steps=None, groups=None, checkpoint_every=5, adjoint_tol=1e-12,
-                 initial=None):
+                 initial=None, store_increments=True):
         if solver.dim != 2 or solver.coupled:
             raise ValueError('the adjoint covers quasi_static_solver_2d')
         if solver.prescribed_increment is None:
@@ -77,7 +82,13 @@ class LoadPathAdjoint:
         self.adjoint_tol = adjoint_tol
         # initial state of the load path (default: the solver's current state)
         self.initial = initial if initial is not None else self._snapshot()
-        self.stats = dict(forward_steps=0, adjoint_steps=0, krylov_iters=[], unconverged_steps=[])
+        # keep every step's converged nodal increment (n_nodes x nb doubles, small next to a particle
+        # state): recomputing a segment from its checkpoint is then a replay (P2G, G2P) with no Newton
+        # solve; without them the segment's Newton solves run again
+        self.store_increments = bool(store_increments)
+        self.increments = {}
+        self.stats = dict(forward_steps=0, adjoint_steps=0, replayed_steps=0, krylov_iters=[],
+                          unconverged_steps=[])
 
     # ------------------------------------------------------------------ state
     def _snapshot(self):
@@ -162,16 +173,32 @@ class LoadPathAdjoint:
         self.stats['forward_steps'] += 1
         return u, f
 
+    def _replay(self, step, u):
+        """Advance one load step with its stored converged increment (no Newton solve):
+        the particle stage at u_k, then G2P.  (The Newton loop commits the stage of its
+        last iterate before the final correction, which differs from u_k by that
+        correction; the replay commits the state at u_k itself, the state the reverse
+        sweep differentiates.)"""
+        s = self.s
+        s.prepare_step(step, self.total_steps)
+        s._inc.copy_(u)
+        s.engine.stage(s._args(step, self.total_steps), want_tangent=False)
+        s.finish_step()
+        self.stats['replayed_steps'] += 1
+
     # ------------------------------------------------------------------ forward
     def forward(self):
         """Run the load path from the initial state; returns forces (n_steps, n_groups)."""
         self._restore(self.initial)
         self.checkpoints = {}
+        self.increments = {}
         forces = torch.zeros(self.n_steps, self.n_groups, dtype=torch.float64, device=self.s.device)
         for k in range(self.n_steps):
             if k % self.k == 0:
                 self.checkpoints[k] = self._snapshot()
-            _, forces[k] = self._step(k)
+            u, forces[k] = self._step(k)
+            if self.store_increments:
+                self.increments[k] = u
         return forces
 
     # ------------------------------------------------------------------ reverse
@@ -255,7 +282,11 @@ class LoadPathAdjoint:
             seg = []
             for k in range(c, end):
                 st = self._snapshot()
-                u, _ = self._step(k)
+                if k in self.increments:
+                    u = self.increments[k]
+                    self._replay(k, u)
+                else:
+                    u, _ = self._step(k)
                 seg.append((k, st, u))
             for k, st, u in reversed(seg):
                 Sbar, th = self._reverse_step(k, st, u, fbar[k], Sbar)

@@ -7,7 +7,10 @@
   1e-4 relative.
 * Batched specimens: torch.autograd through LoadPathFunction gives per-specimen
   gradients equal to central finite differences of the device forward
-  (h = 1e-6 deg / kPa) to 1e-4 relative, and checkpointing does not change them.
+  (h = 1e-6 deg / kPa) to 1e-4 relative, and neither the checkpoint interval
+  nor replaying the stored increments (instead of re-solving the segments)
+  changes them.
+* A load path whose platen rows move repeats exactly from its initial state.
 """
 import os
 
@@ -110,8 +113,9 @@ def test_batched_gradients_match_device_fd_and_checkpointing():
         f_obs = LoadPathFunction.apply(torch.tensor(phi0 + 1.5, device=dev), torch.tensor(coh0 + 2.0, device=dev),
                                        adj_obs)
     grads = []
-    for every in (10, 2):
-        adj = LoadPathAdjoint(s, 4, groups=g, checkpoint_every=every, initial=adj_obs.initial)
+    for every, store in ((10, True), (2, True), (2, False)):
+        adj = LoadPathAdjoint(s, 4, groups=g, checkpoint_every=every, initial=adj_obs.initial,
+                              store_increments=store)
         phi = torch.tensor(phi0, device=dev, requires_grad=True)
         coh = torch.tensor(coh0, device=dev, requires_grad=True)
         f = LoadPathFunction.apply(phi, coh, adj)
@@ -120,6 +124,10 @@ def test_batched_gradients_match_device_fd_and_checkpointing():
         grads.append((phi.grad.cpu().numpy(), coh.grad.cpu().numpy()))
     np.testing.assert_allclose(grads[0][0], grads[1][0], rtol=1e-10)
     np.testing.assert_allclose(grads[0][1], grads[1][1], rtol=1e-10)
+    # replaying stored increments or re-solving the segments gives the same gradients (the two
+    # recomputations differ by the last Newton correction, below 1e-12 relative here)
+    np.testing.assert_allclose(grads[1][0], grads[2][0], rtol=1e-9)
+    np.testing.assert_allclose(grads[1][1], grads[2][1], rtol=1e-9)
 
     def loss_at(ph, co):
         adj = LoadPathAdjoint(s, 4, groups=g, initial=adj_obs.initial)
