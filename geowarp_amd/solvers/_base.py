@@ -17,7 +17,8 @@ Extensions over the reference (all optional, defaults reproduce it):
                                   Drucker-Prager), ['line_search'], ['autotune'],
   ['krylov'] ('mg' default: multigrid-preconditioned flexible GMRES |
   'gmres': block-Jacobi GMRES | 'bicgstab'), ['krylov_restart'],
-  ['mg_nu'], ['mg_omega'], ['mg_coarse_sweeps'], ['mg_ublock']  V-cycle options
+  ['mg_nu'] (both), ['mg_nu_pre'], ['mg_nu_post'] (smoothing sweeps, default 1 and 2),
+  ['mg_omega'] (damping, 0.4), ['mg_coarse_sweeps'], ['mg_ublock']  V-cycle options
   ['mg_lowprec']                  mixed form: FP16 (row-scaled) smoother matrices (default True)
   ['mixed_assembly']              mixed form: 'fp32' tangent and Jacobian (default) or 'fp64'
                                   (FP64 assembly, FP32 copy for the Krylov solve, refinement
@@ -113,7 +114,8 @@ class _QuasiStaticBase:
         self.autotune = it.get('autotune', True)
         self.line_search_steps = []
         # multigrid V-cycle options (krylov='mg'): smoothing sweeps, damping, coarsest-level sweeps
-        mg_opts = {k: it[k2] for k, k2 in (('nu_pre', 'mg_nu'), ('nu_post', 'mg_nu'), ('omega', 'mg_omega'),
+        mg_opts = {k: it[k2] for k, k2 in (('nu_pre', 'mg_nu'), ('nu_post', 'mg_nu'), ('nu_pre', 'mg_nu_pre'),
+                                           ('nu_post', 'mg_nu_post'), ('omega', 'mg_omega'),
                                            ('coarse_sweeps', 'mg_coarse_sweeps'), ('ublock', 'mg_ublock'),
                                            ('lowprec', 'mg_lowprec'))
                    if k2 in it}

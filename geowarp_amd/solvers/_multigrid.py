@@ -55,7 +55,7 @@ class Multigrid:
     for the displacement block of a u-p system (nb - 1 dofs per node) inside the
     block-triangular u-p preconditioner (gm_up_setup / gm_gmres_up)."""
 
-    def __init__(self, eng, fp32, max_levels=8, min_cells=2, nu_pre=3, nu_post=3, coarse_sweeps=20, omega=0.4,
+    def __init__(self, eng, fp32, max_levels=8, min_cells=2, nu_pre=1, nu_post=2, coarse_sweeps=20, omega=0.4,
                  max_dense=3000, ublock=False, gamma=1, lowprec=None):
         self.eng = eng
         self.fp32 = bool(fp32)
