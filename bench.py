@@ -46,6 +46,8 @@ def parse():
     ap.add_argument('--warmup', type=int, default=1, help='untimed load steps before the timed ones')
     ap.add_argument('--cells', type=str, default='64,64,32')
     ap.add_argument('--material', type=str, default='Neo-Hookean')
+    ap.add_argument('--dp-apex', type=str, default='reference',
+                    help="Drucker-Prager apex rule: 'reference' (no return for P_trial >= 0) or 'return'")
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--verbose', action='store_true')
@@ -56,13 +58,13 @@ def parse():
     return ap.parse_args()
 
 
-def build_solver(cells, precision, material, **kw):
+def build_solver(cells, precision, material, tol=1e-8, **kw):
     import geowarp_amd.solvers as gs
     from geowarp_amd import problems as P
     dx = 1.0 / 64
     kw.setdefault('n_iter', 12)
     cfg = P.indent3d(n_cells=cells, dx=dx, total_steps=40, depth=0.05, footing=0.2 * cells[0] / 64.0,
-                     material=material, precision=precision, tol=1e-8, **kw)
+                     material=material, precision=precision, tol=tol, **kw)
     s = P.build_solver(gs, cfg, device='cuda:0')
     cfg['particle_flags'](s.x_particles, s.particle_traction_flag_array, s.particle_pressure_boundary_flag_array)
     s.line_search = True
@@ -305,7 +307,7 @@ def main():
     forms = ['mixed'] if args.no_fp64 else ['fp64', 'mixed']
     solvers = {}
     for form in forms:
-        s, cfg = build_solver(cells, form, args.material, krylov=args.krylov)
+        s, cfg = build_solver(cells, form, args.material, krylov=args.krylov, dp_apex=args.dp_apex)
         solvers[form] = s
         run_steps(s, cfg, 0, args.warmup)
         s.engine.timers = PhaseTimer()
@@ -358,6 +360,15 @@ def main():
     if 'fp64' in res:
         out['mixed_vs_fp64_speedup'] = round(res['fp64']['seconds'] / res['mixed']['seconds'], 3)
         out['mixed_vs_fp64_field_rel_err'] = field_errors(solvers['mixed'], solvers['fp64'])
+        # the same load steps in the mixed form with Newton run to 1e-11 (untimed): the field
+        # differences above are set by the 1e-8 Newton tolerance, these by the precision
+        st, cfgt = build_solver(cells, 'mixed', args.material, tol=1e-11, krylov=args.krylov, dp_apex=args.dp_apex)
+        run_steps(st, cfgt, 0, args.warmup + args.steps)
+        h = st.residual_history
+        out['mixed_tol_1e-11_vs_fp64_field_rel_err'] = dict(
+            field_errors(st, solvers['fp64']), newton_iters_last_step=st.newton_iterations,
+            relative_residual=h[-1] / h[0] if h and h[0] > 0 else None)
+        del st
     if args.c2_steps > 0:
         out['c2_j2_bar'] = c2_bench(args.c2_steps)
     if args.adjoint_steps > 0:
