@@ -55,6 +55,8 @@ def parse():
                     help='load steps of the batched-specimen adjoint measurement (0 = skip)')
     ap.add_argument('--krylov', type=str, default='mg', help="'mg' (multigrid GMRES) or 'gmres' (block Jacobi)")
     ap.add_argument('--c2-steps', type=int, default=2, help='load steps of the 2D J2 bar measurement (0 = skip)')
+    ap.add_argument('--c3-steps', type=int, default=101,
+                    help='load steps of the 2D consolidation column measurement (0 = skip)')
     return ap.parse_args()
 
 
@@ -236,6 +238,69 @@ def c2_bench(steps):
     return out
 
 
+def _terzaghi_series(z, H, Tv, n_terms=400):
+    """Terzaghi's excess pore pressure u/u0 at depth z below a drained top, impermeable
+    base at depth H: sum_m 2/M sin(M z/H) exp(-M^2 Tv), M = pi (2m+1)/2."""
+    M = np.pi * (2 * np.arange(n_terms) + 1) / 2
+    return (2 / M[None, :] * np.sin(M[None, :] * z[:, None] / H) * np.exp(-M[None, :] ** 2 * Tv)).sum(1)
+
+
+def c3_bench(steps):
+    """2D u-p consolidation column (Terzaghi), 32 x 320 cells of 1/32 m, 40,960
+    particles, Neo-Hookean skeleton E 10 MPa, nu 0.3, mobility 1e-4, drained top,
+    impermeable base: a 10 kPa surface load in load step 0 (dt 1e-3 s, undrained),
+    then `steps - 1` consolidation steps at log-spaced times up to t = 150 s
+    (time factor T_v ~ 2), Newton to 1e-8 relative residual.  FP64 and mixed forms
+    from the same initial state: times, Newton iterations, field errors of the mixed
+    form; and the FP64 pore pressure against Terzaghi's series solution scaled by the
+    undrained pressure of load step 0, at T_v ~ 0.05, 0.2, 0.5 and 1."""
+    import torch
+    import geowarp_amd.solvers as gs
+    from geowarp_amd import problems as P
+    t_end, t1 = 150.0, 0.05
+    tk = np.concatenate([[1e-3], 1e-3 + np.logspace(np.log10(t1), np.log10(t_end), steps - 1)])
+    dts = np.diff(np.concatenate([[0.0], tk]))
+    H, E, nu, mob = 10.0, 1.0e4, 0.3, 1e-4
+    cv = mob * E * (1 - nu) / ((1 + nu) * (1 - 2 * nu))  # consolidation coefficient, m^2/s
+    Tv = cv * (tk - tk[0]) / H ** 2
+    targets = [0.05, 0.2, 0.5, 1.0]
+    check_steps = {int(np.argmin(np.abs(Tv - t))): t for t in targets if Tv[-1] >= t}
+    out, solvers, p0, series = {}, {}, None, []
+    for form in ('fp64', 'mixed'):
+        cfg = P.terzaghi(precision=form, total_steps=steps, tol=1e-8)
+        s = P.build_solver(gs, cfg, device='cuda:0')
+        cfg['particle_flags'](s.x_particles, s.particle_traction_flag_array, s.particle_pressure_boundary_flag_array)
+        z = H - s.x0[:, 1].cpu().numpy()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        iters, unconverged = 0, 0
+        for k in range(steps):
+            s.dt = float(dts[k])
+            s.advance_one_step(k, steps)
+            iters += s.newton_iterations
+            unconverged += int(not s.newton_converged)
+            if form == 'fp64':
+                if k == 0:
+                    p0 = float(s.particle_pressure_array.mean())
+                if k in check_steps:
+                    pn = s.particle_pressure_array.cpu().numpy()
+                    ua = _terzaghi_series(z, H, Tv[k]) * p0
+                    series.append(dict(step=k, time_factor=round(float(Tv[k]), 4),
+                                       rel_l2=float(np.linalg.norm(pn - ua) / np.linalg.norm(ua)),
+                                       base_kPa=dict(mpm=float(pn[np.argmax(z)]), series=float(ua[np.argmax(z)]))))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[form] = dict(seconds=round(dt, 3), newton_iters=iters, unconverged_steps=unconverged)
+        solvers[form] = s
+    a, b = solvers['mixed'], solvers['fp64']
+    out['mixed_vs_fp64_speedup'] = round(out['fp64']['seconds'] / out['mixed']['seconds'], 3)
+    out['mixed_vs_fp64_field_rel_err'] = field_errors(a, b)
+    out['fp64_vs_terzaghi'] = dict(undrained_pressure_kPa=p0, consolidation_coefficient=cv, checks=series)
+    out['workload'] = ('2D u-p consolidation column 32x320 cells, %d particles, %d load steps (log-spaced times to '
+                       't = %g s) to 1e-8 relative residual' % (b.n_particles, steps, t_end))
+    return out
+
+
 def measured_traffic(cells, material):
     """HBM bytes per launch of the level-0 FP32 SpMV from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/k_spmv_c4_indentation*),
@@ -371,6 +436,8 @@ def main():
         del st
     if args.c2_steps > 0:
         out['c2_j2_bar'] = c2_bench(args.c2_steps)
+    if args.c3_steps > 0:
+        out['c3_consolidation'] = c3_bench(args.c3_steps)
     if args.adjoint_steps > 0:
         out['adjoint'] = adjoint_bench(args.adjoint_steps, krylov=args.krylov)
     if not args.no_cpu:

@@ -1263,14 +1263,18 @@ __global__ void k_jac_update(int n_rows, const T* dinv, const T* b, const T* ax,
   }
 }
 
-// x = Minv b for the coarsest level's dense inverse (one row per thread)
+// x = Minv b for the coarsest level's dense inverse: one wavefront per row, lanes
+// striding the row (coalesced), fixed-order shuffle reduction
 template <typename T>
-__global__ void k_dense_mv(int n, const double* Minv, const T* b, T* x) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ void __launch_bounds__(256) k_dense_mv(int n, const double* Minv, const T* b, T* x) {
+  int i = blockIdx.x * 4 + threadIdx.x / 64;
+  int lane = threadIdx.x % 64;
   double s = 0.0;
-  for (int j = 0; j < n; ++j) s += (double)Minv[(long)i * n + j] * (double)b[j];
-  x[i] = (T)s;
+  if (i < n)
+    for (int j = lane; j < n; j += 64) s += Minv[(long)i * n + j] * (double)b[j];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (i < n && lane == 0) x[i] = (T)s;
 }
 
 template <typename T>
@@ -1327,7 +1331,7 @@ static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
   bool coarsest = (l == cy.n_levels - 1);
   if (coarsest && lv.dense_inv) {
     int nd = n * NB;
-    k_dense_mv<T><<<gm_blocks(nd, 128), 128, 0, st>>>(nd, (const double*)lv.dense_inv, b, x);
+    k_dense_mv<T><<<gm_blocks(nd, 4), 256, 0, st>>>(nd, (const double*)lv.dense_inv, b, x);
     GM_LAUNCH_CHECK();
     return 0;
   }

@@ -18,7 +18,7 @@ Extensions over the reference (all optional, defaults reproduce it):
   ['krylov'] ('mg' default: multigrid-preconditioned flexible GMRES |
   'gmres': block-Jacobi GMRES | 'bicgstab'), ['krylov_restart'],
   ['mg_nu'] (both), ['mg_nu_pre'], ['mg_nu_post'] (smoothing sweeps, default 1 and 2),
-  ['mg_omega'] (damping, 0.4), ['mg_coarse_sweeps'], ['mg_ublock']  V-cycle options
+  ['mg_omega'] (damping, 0.4), ['mg_coarse_sweeps'], ['mg_ublock'], ['mg_min_cells']  V-cycle options
   ['mg_lowprec']                  mixed form: FP16 (row-scaled) smoother matrices (default True)
   ['mixed_assembly']              mixed form: 'fp32' tangent and Jacobian (default) or 'fp64'
                                   (FP64 assembly, FP32 copy for the Krylov solve, refinement
@@ -117,7 +117,7 @@ class _QuasiStaticBase:
         mg_opts = {k: it[k2] for k, k2 in (('nu_pre', 'mg_nu'), ('nu_post', 'mg_nu'), ('nu_pre', 'mg_nu_pre'),
                                            ('nu_post', 'mg_nu_post'), ('omega', 'mg_omega'),
                                            ('coarse_sweeps', 'mg_coarse_sweeps'), ('ublock', 'mg_ublock'),
-                                           ('lowprec', 'mg_lowprec'))
+                                           ('lowprec', 'mg_lowprec'), ('min_cells', 'mg_min_cells'))
                    if k2 in it}
         self.krylov_restart = it.get('krylov_restart', 250 if self.krylov_method == 'mg' else 40)
         # inner tolerance: 'fixed' (linear_tol) or 'ew' (Eisenstat-Walker choice 2)
@@ -131,6 +131,9 @@ class _QuasiStaticBase:
         # at relative residual 3e-2 with a 1e-2 cap in FP64 as in FP32, and converges
         # quadratically with 1e-4)
         self.eta_max = it.get('forcing_max', 1e-4 if self.material_name in ('J2', 'Drucker-Prager') else 1e-2)
+        # Eisenstat-Walker choice 2 parameters: eta_k = gamma (|r_k| / |r_k-1|)^alpha
+        self.ew_gamma = it.get('forcing_gamma', 0.9)
+        self.ew_alpha = it.get('forcing_alpha', 2.0)
         if self.material_name not in MAT_KIND:
             raise ValueError("Unimplemented material model: %s" % self.material_name)
         self.youngs_modulus = m['youngs_modulus']
@@ -321,7 +324,7 @@ class _QuasiStaticBase:
         """Eisenstat & Walker (1996) choice 2 forcing term with their safeguards,
         clamped to [linear_tol, forcing_max], and not tighter than needed to
         reach the Newton tolerance."""
-        gamma, alpha = 0.9, 2.0
+        gamma, alpha = self.ew_gamma, self.ew_alpha
         if nr_prev is None or eta_prev is None:
             eta = min(1e-2, self.eta_max)
         else:
