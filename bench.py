@@ -287,6 +287,7 @@ def c3_bench(steps):
                     ua = _terzaghi_series(z, H, Tv[k]) * p0
                     series.append(dict(step=k, time_factor=round(float(Tv[k]), 4),
                                        rel_l2=float(np.linalg.norm(pn - ua) / np.linalg.norm(ua)),
+                                       max_abs_over_p0=float(np.abs(pn - ua).max() / p0),
                                        base_kPa=dict(mpm=float(pn[np.argmax(z)]), series=float(ua[np.argmax(z)]))))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
