@@ -14,12 +14,16 @@ Extensions over the reference (all optional, defaults reproduce it):
   iteration_dict['linear_tol'], ['linear_maxiter'], ['ir_steps'], ['forcing']
                                   ('fixed' | 'ew': Eisenstat-Walker inner tolerance),
   ['forcing_max']                 cap of the 'ew' inner tolerance (1e-2; 1e-4 for J2 and
-                                  Drucker-Prager), ['line_search'], ['autotune'],
+                                  Drucker-Prager; a hundred times tighter after each Newton
+                                  iteration that reduces the residual by less than half),
+  ['line_search'], ['autotune'],
   ['krylov'] ('mg' default: multigrid-preconditioned flexible GMRES |
   'gmres': block-Jacobi GMRES | 'bicgstab'), ['krylov_restart'],
   ['mg_nu'] (both), ['mg_nu_pre'], ['mg_nu_post'] (smoothing sweeps, default 1 and 2),
   ['mg_omega'] (damping, 0.4), ['mg_coarse_sweeps'], ['mg_ublock'], ['mg_min_cells']  V-cycle options
   ['mg_lowprec']                  mixed form: FP16 (row-scaled) smoother matrices (default True)
+  ['mixed_fallback']              mixed form: re-solve a correction in FP64 (FP64 tangent, Jacobian
+                                  and hierarchy) when the FP32 solve leaves relres > 0.1 (default True)
   ['mixed_assembly']              mixed form: 'fp32' tangent and Jacobian (default) or 'fp64'
                                   (FP64 assembly, FP32 copy for the Krylov solve, refinement
                                   residual against the FP64 Jacobian)
@@ -131,6 +135,8 @@ class _QuasiStaticBase:
         # at relative residual 3e-2 with a 1e-2 cap in FP64 as in FP32, and converges
         # quadratically with 1e-4)
         self.eta_max = it.get('forcing_max', 1e-4 if self.material_name in ('J2', 'Drucker-Prager') else 1e-2)
+        # mixed form: solve a Newton correction again in FP64 when the FP32 solve fails
+        self.mixed_fallback = it.get('mixed_fallback', True)
         # Eisenstat-Walker choice 2 parameters: eta_k = gamma (|r_k| / |r_k-1|)^alpha
         self.ew_gamma = it.get('forcing_gamma', 0.9)
         self.ew_alpha = it.get('forcing_alpha', 2.0)
@@ -267,10 +273,27 @@ class _QuasiStaticBase:
             nr = math.sqrt(max(eng.scal[0].item(), 0.0))
             tol_lin = self.linear_tol
             if self.forcing == 'ew':
+                if nr_prev is None:
+                    self._eta_cap = self.eta_max
+                elif nr > 0.5 * nr_prev:
+                    # the iteration stalls: later corrections are solved a hundred times tighter
+                    self._eta_cap = max(self.linear_tol, 1e-2 * self._eta_cap)
                 eta = self._ew_forcing(nr, nr_prev, eta, r0 if r0 is not None else nr)
                 tol_lin = eta
             eng.solve(tol_lin, self.linear_maxiter, ir_steps=self.ir_steps, method=self.krylov_method,
                       restart=self.krylov_restart, equilibrate=self.equilibrate)
+            if (eng.precision == 'mixed' and self.mixed_fallback and not self.equilibrate
+                    and not (eng.stats['krylov_relres'][-1] <= 0.1)):
+                # the FP32 solve did not even reduce the residual tenfold: assemble the tangent and
+                # the Jacobian in FP64 at the same iterate and solve this correction in FP64
+                prev = eng.mixed_assembly
+                eng.mixed_assembly = 'fp64'
+                try:
+                    eng.stage(args, want_tangent=True)
+                    eng.jacobian(x, lp)
+                finally:
+                    eng.mixed_assembly = prev
+                eng.solve_fp64(tol_lin, self.linear_maxiter, method=self.krylov_method, restart=self.krylov_restart)
             if tm is not None:
                 tm.mark('krylov')
             converged_here = (nr if self.tol_mode == 'absolute' else (nr / r0 if r0 else 1.0)) < self.tol
@@ -335,7 +358,7 @@ class _QuasiStaticBase:
         target = self.tol * r0 if self.tol_mode == 'relative' else self.tol
         if nr > 0:
             eta = max(eta, 0.5 * target / nr)
-        return min(max(eta, self.linear_tol), self.eta_max)
+        return min(max(eta, self.linear_tol), getattr(self, '_eta_cap', self.eta_max))
 
     # ------------------------------------------------------------------ dof-major views
     def _sync_flags_out(self):

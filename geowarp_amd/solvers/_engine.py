@@ -422,6 +422,26 @@ class Engine:
         self.stats['krylov_iters'].append(total)
         self.stats['krylov_relres'].append(rel)
 
+    def solve_fp64(self, tol, maxiter, method='mg', restart=250, check_every=8):
+        """Mixed form, fallback: solve K delta = rhs in FP64 on the FP64 Jacobian
+        (vals64, from an FP64 assembly) with an FP64 hierarchy; replaces the last
+        solve's statistics."""
+        if self.vals64 is None:
+            raise L.GMError('solve_fp64: no FP64 Jacobian assembled')
+        saved = self.vals
+        self.vals = self.vals64
+        self.jac_version += 1  # hierarchies are rebuilt for the FP64 matrix (and again after)
+        try:
+            self.delta.zero_()
+            its, rel = self.krylov(self.rhs, self.delta, False, tol, maxiter, method, restart, check_every)
+        finally:
+            self.vals = saved
+            self.jac_version += 1
+        self.stats['krylov_iters'][-1] += its
+        self.stats['krylov_relres'][-1] = rel
+        self.stats['fp64_fallbacks'] = self.stats.get('fp64_fallbacks', 0) + 1
+        return its, rel
+
     def axpy(self, a, x, y):
         L.check(L.lib().gm_axpy(x.numel(), float(a), L.ptr(x), int(x.dtype == torch.float32), L.ptr(y), _stream()),
                 'gm_axpy')
