@@ -3,7 +3,9 @@ norm test and iterative refinement) against the FP64 oracle fixtures.
 
 The mixed form converges the same FP64 residual to the fixtures' relative
 tolerance, so its fields must agree with the oracle to within what that
-tolerance allows: 1e-6 relative (the FP64 form is held to 1e-8).
+tolerance allows: 1e-6 relative (the FP64 form is held to 1e-8).  The same bar
+holds with the FP64 assembly option and when every correction falls back to an
+FP64 solve.
 """
 import os
 
@@ -16,14 +18,14 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
 CASES = ['bar_nh', 'bar_j2', 'bar_j2_platen', 'bar_dp', 'terzaghi', 'indent_nh']
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize('name', CASES)
-def test_mixed_form_matches_oracle(name):
+def _run_against_fixture(name, extra, engine_setup=None):
     import geowarp_amd.solvers as gs
     g = np.load(os.path.join(GOLD, name + '.npz'))
     cfg = H.MINI[name]()
-    cfg['iteration_dict'] = dict(cfg['iteration_dict'], precision='mixed')
+    cfg['iteration_dict'] = dict(cfg['iteration_dict'], precision='mixed', **extra)
     s = H.make(gs, cfg)
+    if engine_setup is not None:
+        engine_setup(s.engine)
     total = int(g['total_steps'][0])
     x0 = s.x0.cpu().numpy()
     for k in range(int(g['n_steps'][0])):
@@ -41,3 +43,29 @@ def test_mixed_form_matches_oracle(name):
                 e = H.rel_err(f[key], g[gk])
             print('   %-6s rel err %.3e' % (key, e))
             assert e < 1e-6, (name, k, key, e)
+    return s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', CASES)
+def test_mixed_form_matches_oracle(name):
+    _run_against_fixture(name, {})
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', ['bar_j2', 'terzaghi', 'indent_nh'])
+def test_mixed_form_with_fp64_assembly_matches_oracle(name):
+    s = _run_against_fixture(name, dict(mixed_assembly='fp64'))
+    assert s.engine.vals64 is not None
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', ['bar_j2', 'indent_nh'])
+def test_mixed_form_fp64_fallback_matches_oracle(name):
+    """FP32 solves capped at one Krylov iteration leave relres > 0.1, so every
+    correction is solved again in FP64 (FP64 tangent, Jacobian and hierarchy)."""
+    def cap(eng):
+        eng.mixed_inner_maxiter = 1
+    s = _run_against_fixture(name, dict(ir_steps=1), cap)
+    print(name, 'fallbacks', s.engine.stats.get('fp64_fallbacks'))
+    assert s.engine.stats.get('fp64_fallbacks', 0) > 0
