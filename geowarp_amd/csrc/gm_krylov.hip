@@ -159,15 +159,40 @@ __global__ void __launch_bounds__(T) k_lo_convert(int n_rows, const int* row_ptr
     for (int q = 0; q < NB * NB; ++q) lo[(long)k * NB * NB + q] = (gm_half)((float)vals[(long)k * NB * NB + q] * inv[q]);
 }
 
-template <int NB, int G, int T>
-__global__ void __launch_bounds__(T) k_spmv_lo(int n_rows, const int* row_ptr, const int* col_idx,
-                                               const gm_half* lo, const float* scale, const float* x, float* y) {
-  const int rows_per_blk = T / G;
-  int row = blockIdx.x * rows_per_blk + threadIdx.x / G;
-  int lane = threadIdx.x % G;
-  float acc[NB * NB];  // per (r, c): sum over the row's blocks of lo[r][c] x[c]
+// Row product of a level matrix, 16 (8, 4) lanes per block row; every lane of the row
+// ends with the row's full product (butterfly reduction in a fixed order).
+template <int NB, int G, typename TM, typename TX>
+__device__ __forceinline__ void row_acc(int row, int lane, int n_rows, const int* row_ptr, const int* col_idx,
+                                        const TM* vals, const TX* x, double acc[NB]) {
 #pragma unroll
-  for (int q = 0; q < NB * NB; ++q) acc[q] = 0.0f;
+  for (int r = 0; r < NB; ++r) acc[r] = 0.0;
+  if (row < n_rows) {
+    int s = row_ptr[row], e = row_ptr[row + 1];
+    for (int k = s + lane; k < e; k += G) {
+      int c = col_idx[k];
+      const TM* blk = vals + (long)k * NB * NB;
+      double xv[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) xv[j] = (double)x[(long)c * NB + j];
+#pragma unroll
+      for (int r = 0; r < NB; ++r)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[r] += (double)blk[r * NB + j] * xv[j];
+    }
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int r = 0; r < NB; ++r) acc[r] += __shfl_xor(acc[r], off, G);
+}
+
+// The same from the FP16 copy: FP32 partial sums per (r, c), scaled after the reduction.
+template <int NB, int G>
+__device__ __forceinline__ void row_acc_lo(int row, int lane, int n_rows, const int* row_ptr, const int* col_idx,
+                                           const gm_half* lo, const float* scale, const float* x, double acc[NB]) {
+  float a[NB * NB];
+#pragma unroll
+  for (int q = 0; q < NB * NB; ++q) a[q] = 0.0f;
   if (row < n_rows) {
     int s = row_ptr[row], e = row_ptr[row + 1];
     for (int k = s + lane; k < e; k += G) {
@@ -179,21 +204,54 @@ __global__ void __launch_bounds__(T) k_spmv_lo(int n_rows, const int* row_ptr, c
 #pragma unroll
       for (int r = 0; r < NB; ++r)
 #pragma unroll
-        for (int j = 0; j < NB; ++j) acc[r * NB + j] += (float)blk[r * NB + j] * xv[j];
+        for (int j = 0; j < NB; ++j) a[r * NB + j] += (float)blk[r * NB + j] * xv[j];
     }
   }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1)
 #pragma unroll
-    for (int q = 0; q < NB * NB; ++q) acc[q] += __shfl_xor(acc[q], off, G);
-  if (row < n_rows && lane == 0)
+    for (int q = 0; q < NB * NB; ++q) a[q] += __shfl_xor(a[q], off, G);
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    double v = 0.0;
+    if (row < n_rows)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) v += (double)scale[((long)row * NB + r) * NB + j] * (double)a[r * NB + j];
+    acc[r] = v;
+  }
+}
+
+// One smoother-side product of a multigrid level, fused with what consumes it:
+//   MODE 1 (residual):      out = b - A x
+//   MODE 2 (Jacobi sweep):  out = x + omega Dinv (b - A x)      (out != x)
+// A is the level's matrix (TM) or, with LO, its FP16 copy (FP32 hierarchies).
+template <int NB, int G, int MODE, typename TM, typename T, bool LO>
+__global__ void __launch_bounds__(256) k_level(int n_rows, const int* row_ptr, const int* col_idx, const TM* vals,
+                                               const gm_half* lo, const float* scale, const T* x, const T* b,
+                                               const T* dinv, double omega, T* out) {
+  int row = blockIdx.x * (256 / G) + threadIdx.x / G;
+  int lane = threadIdx.x % G;
+  double acc[NB];
+  if constexpr (LO)
+    row_acc_lo<NB, G>(row, lane, n_rows, row_ptr, col_idx, lo, scale, (const float*)x, acc);
+  else
+    row_acc<NB, G, TM, T>(row, lane, n_rows, row_ptr, col_idx, vals, x, acc);
+  if (row >= n_rows || lane != 0) return;
+  double res[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) res[r] = (double)b[(long)row * NB + r] - acc[r];
+  if (MODE == 1) {
+#pragma unroll
+    for (int r = 0; r < NB; ++r) out[(long)row * NB + r] = (T)res[r];
+  } else {
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
-      float v = 0.0f;
+      double s = 0.0;
 #pragma unroll
-      for (int j = 0; j < NB; ++j) v += scale[((long)row * NB + r) * NB + j] * acc[r * NB + j];
-      y[(long)row * NB + r] = v;
+      for (int j = 0; j < NB; ++j) s += (double)dinv[(long)row * NB * NB + r * NB + j] * res[j];
+      out[(long)row * NB + r] = (T)((double)x[(long)row * NB + r] + omega * s);
     }
+  }
 }
 
 template <int NB, typename TM>
@@ -201,13 +259,6 @@ static void launch_lo_convert(int n_rows, const int* rp, const TM* vals, gm_half
                               hipStream_t s) {
   constexpr int G = spmv_lanes(NB);
   k_lo_convert<NB, G, 256, TM><<<gm_blocks(n_rows, 256 / G), 256, 0, s>>>(n_rows, rp, vals, lo, scale);
-}
-
-template <int NB>
-static void launch_spmv_lo(int n_rows, const int* rp, const int* ci, const gm_half* lo, const float* scale,
-                           const float* x, float* y, hipStream_t s) {
-  constexpr int G = spmv_lanes(NB);
-  k_spmv_lo<NB, G, 256><<<gm_blocks(n_rows, 256 / G), 256, 0, s>>>(n_rows, rp, ci, lo, scale, x, y);
 }
 
 // workgroups of a launch with fused dot products (SPMV_T threads): the partial-sum count
@@ -1246,23 +1297,6 @@ __global__ void k_jac_first(int n_rows, const T* dinv, const T* b, double omega,
   }
 }
 
-// x += omega Dinv (b - Ax)
-template <int NB, typename T>
-__global__ void k_jac_update(int n_rows, const T* dinv, const T* b, const T* ax, double omega, T* x) {
-  int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n_rows) return;
-  double v[NB];
-#pragma unroll
-  for (int j = 0; j < NB; ++j) v[j] = (double)b[(long)row * NB + j] - (double)ax[(long)row * NB + j];
-#pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    double s = 0.0;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) s += (double)dinv[(long)row * NB * NB + r * NB + j] * v[j];
-    x[(long)row * NB + r] = (T)((double)x[(long)row * NB + r] + omega * s);
-  }
-}
-
 // x = Minv b for the coarsest level's dense inverse: one wavefront per row, lanes
 // striding the row (coalesced), fixed-order shuffle reduction
 template <typename T>
@@ -1283,11 +1317,6 @@ __global__ void k_axpy_t(long n, const T* a, T* x) {
     x[i] = (T)((double)x[i] + (double)a[i]);
 }
 
-template <typename T>
-__global__ void k_sub_inplace(long n, const T* b, T* r) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    r[i] = (T)((double)b[i] - (double)r[i]);
-}
 
 inline Dims dims_of(const gm_mg_level& l) {
   Dims d;
@@ -1303,22 +1332,46 @@ struct Cycle {
   const gm_mg_level* L;
 };
 
-// y = A x on level l, from the FP16 copy where the level has one (FP32 hierarchies);
-// level 0 holds the Jacobian in the solve precision, coarser levels are FP64
-template <int NB, typename T>
-static void level_spmv(const gm_mg_level& lv, int l, const T* x, T* y, hipStream_t st) {
+// A fused level operation (k_level MODE) on level l: from the FP16 copy where the level has
+// one (FP32 hierarchies); level 0 holds the Jacobian in the solve precision, coarser levels FP64
+template <int NB, int MODE, typename T>
+static void level_op(const gm_mg_level& lv, int l, const T* x, const T* b, const T* dinv, double omega, T* out,
+                     hipStream_t st) {
+  constexpr int G = spmv_lanes(NB);
+  int blocks = gm_blocks(lv.n_nodes, 256 / G);
   if constexpr (std::is_same<T, float>::value) {
     if (lv.vals_lo) {
-      launch_spmv_lo<NB>(lv.n_nodes, lv.row_ptr, lv.col_idx, (const gm_half*)lv.vals_lo, lv.row_scale, x, y, st);
+      k_level<NB, G, MODE, float, float, true><<<blocks, 256, 0, st>>>(
+          lv.n_nodes, lv.row_ptr, lv.col_idx, nullptr, (const gm_half*)lv.vals_lo, lv.row_scale, x, b, dinv, omega,
+          out);
       return;
     }
   }
   if (l == 0)
-    launch_spmv<NB, T, T, T>(lv.n_nodes, lv.row_ptr, lv.col_idx, (const T*)lv.vals, x, y, nullptr, nullptr,
-                             nullptr, st);
+    k_level<NB, G, MODE, T, T, false><<<blocks, 256, 0, st>>>(lv.n_nodes, lv.row_ptr, lv.col_idx, (const T*)lv.vals,
+                                                               nullptr, nullptr, x, b, dinv, omega, out);
   else
-    launch_spmv<NB, double, T, T>(lv.n_nodes, lv.row_ptr, lv.col_idx, (const double*)lv.vals, x, y, nullptr,
-                                  nullptr, nullptr, st);
+    k_level<NB, G, MODE, double, T, false><<<blocks, 256, 0, st>>>(lv.n_nodes, lv.row_ptr, lv.col_idx,
+                                                                    (const double*)lv.vals, nullptr, nullptr, x, b,
+                                                                    dinv, omega, out);
+}
+
+// `count` damped block-Jacobi sweeps on x, alternating between x and the level's work
+// vector r (a sweep cannot update in place); the result ends in x.
+template <int NB, typename T>
+static int smooth(const gm_mg_level& lv, int l, const T* b, T* x, int count, double omega, hipStream_t st) {
+  T* src = x;
+  T* dst = (T*)lv.r;
+  const T* dinv = (const T*)lv.dinv;
+  for (int k = 0; k < count; ++k) {
+    level_op<NB, 2, T>(lv, l, src, b, dinv, omega, dst, st);
+    T* t = src;
+    src = dst;
+    dst = t;
+  }
+  GM_LAUNCH_CHECK();
+  if (src != x) GM_HIP(hipMemcpyAsync(x, src, sizeof(T) * (size_t)lv.n_nodes * NB, hipMemcpyDeviceToDevice, st));
+  return 0;
 }
 
 template <int DIM, int NB, typename T>
@@ -1337,31 +1390,22 @@ static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
   }
   int sweeps = coarsest ? cy.coarse_sweeps : cy.nu_pre;
   k_jac_first<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, cy.omega, x);
-  for (int k = 1; k < sweeps; ++k) {
-    level_spmv<NB, T>(lv, l, x, r, st);
-    k_jac_update<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, r, cy.omega, x);
-  }
   GM_LAUNCH_CHECK();
+  int rc = smooth<NB, T>(lv, l, b, x, sweeps - 1, cy.omega, st);
+  if (rc) return rc;
   if (coarsest) return 0;
   const gm_mg_level& cv = cy.L[l + 1];
   const int gamma = lv.gamma > 1 ? lv.gamma : 1;  // 1: V-cycle, 2: W-cycle at this level
   for (int g = 0; g < gamma; ++g) {
-    level_spmv<NB, T>(lv, l, x, r, st);
-    long nv = (long)n * NB;
-    k_sub_inplace<T><<<dot_blocks(nv), RED_T, 0, st>>>(nv, b, r);
+    level_op<NB, 1, T>(lv, l, x, b, nullptr, 0.0, r, st);  // r = b - A x
     k_restrict<DIM, NB, T><<<gm_blocks(cv.n_nodes, 256), 256, 0, st>>>(dims_of(lv), dims_of(cv), cv.n_nodes,
                                                                         lv.freem, r, (T*)cv.b);
     GM_LAUNCH_CHECK();
-    int rc = vcycle<DIM, NB, T>(cy, l + 1, (const T*)cv.b, (T*)cv.x, st);
+    rc = vcycle<DIM, NB, T>(cy, l + 1, (const T*)cv.b, (T*)cv.x, st);
     if (rc) return rc;
     k_prolong_add<DIM, NB, T><<<rb, 256, 0, st>>>(dims_of(lv), dims_of(cv), n, lv.freem, (const T*)cv.x, x);
   }
-  for (int k = 0; k < cy.nu_post; ++k) {
-    level_spmv<NB, T>(lv, l, x, r, st);
-    k_jac_update<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, r, cy.omega, x);
-  }
-  GM_LAUNCH_CHECK();
-  return 0;
+  return smooth<NB, T>(lv, l, b, x, cy.nu_post, cy.omega, st);
 }
 
 template <int DIM, int NB, typename T>
