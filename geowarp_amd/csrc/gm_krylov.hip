@@ -1222,14 +1222,14 @@ __global__ void k_rap(Dims f, Dims c, int n_coarse, const int* rp_c, const uint6
 
 // b_c = P^T r_f (free fine dofs only)
 template <int DIM, int NB, typename T>
+// one thread per coarse dof (consecutive threads: the dofs of a node, then the next node)
 __global__ void k_restrict(Dims f, Dims c, int n_coarse, const uint8_t* freem_f, const T* r_f, T* b_c) {
-  int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= n_coarse) return;
+  long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (gid >= (long)n_coarse * NB) return;
+  int I = (int)(gid / NB), k = (int)(gid % NB);
   int A[3];
   coords<DIM>(I, c.n[0], c.n[1], A);
-  double acc[NB];
-#pragma unroll
-  for (int k = 0; k < NB; ++k) acc[k] = 0.0;
+  double acc = 0.0;
   const int oz = (DIM == 3) ? 1 : 0;
   for (int dz = -oz; dz <= oz; ++dz)
     for (int dy = -1; dy <= 1; ++dy)
@@ -1239,19 +1239,18 @@ __global__ void k_restrict(Dims f, Dims c, int n_coarse, const uint8_t* freem_f,
         if (DIM == 3 && (a[2] < 0 || a[2] > f.n[2])) continue;
         double w = (dx == 0 ? 1.0 : 0.5) * (dy == 0 ? 1.0 : 0.5) * ((DIM == 3 && dz != 0) ? 0.5 : 1.0);
         int i = node_of<DIM>(a, f.n[0], f.n[1]);
-#pragma unroll
-        for (int k = 0; k < NB; ++k)
-          if (freem_f[(long)i * NB + k]) acc[k] += w * (double)r_f[(long)i * NB + k];
+        if (freem_f[(long)i * NB + k]) acc += w * (double)r_f[(long)i * NB + k];
       }
-#pragma unroll
-  for (int k = 0; k < NB; ++k) b_c[(long)I * NB + k] = (T)acc[k];
+  b_c[gid] = (T)acc;
 }
 
-// x_f += P x_c (free fine dofs only)
+// x_f += P x_c (free fine dofs only); one thread per fine dof
 template <int DIM, int NB, typename T>
 __global__ void k_prolong_add(Dims f, Dims c, int n_fine, const uint8_t* freem_f, const T* x_c, T* x_f) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_fine) return;
+  long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (gid >= (long)n_fine * NB) return;
+  int i = (int)(gid / NB), k = (int)(gid % NB);
+  if (!freem_f[gid]) return;
   int a[3];
   coords<DIM>(i, f.n[0], f.n[1], a);
   int lo[3], hi[3];
@@ -1260,9 +1259,7 @@ __global__ void k_prolong_add(Dims f, Dims c, int n_fine, const uint8_t* freem_f
     lo[ax] = a[ax] >> 1;
     hi[ax] = (a[ax] + 1) >> 1;  // == lo for even a
   }
-  double acc[NB];
-#pragma unroll
-  for (int k = 0; k < NB; ++k) acc[k] = 0.0;
+  double acc = 0.0;
   const int zc = (DIM == 3) ? 2 : 1;
   for (int pz = 0; pz < zc; ++pz)
     for (int py = 0; py < 2; ++py)
@@ -1272,29 +1269,22 @@ __global__ void k_prolong_add(Dims f, Dims c, int n_fine, const uint8_t* freem_f
         if ((px && hi[0] == lo[0]) || (py && hi[1] == lo[1]) || (DIM == 3 && pz && hi[2] == lo[2])) continue;
         double w = ((a[0] & 1) ? 0.5 : 1.0) * ((a[1] & 1) ? 0.5 : 1.0) * ((DIM == 3 && (a[2] & 1)) ? 0.5 : 1.0);
         int J = node_of<DIM>(A, c.n[0], c.n[1]);
-#pragma unroll
-        for (int k = 0; k < NB; ++k) acc[k] += w * (double)x_c[(long)J * NB + k];
+        acc += w * (double)x_c[(long)J * NB + k];
       }
-#pragma unroll
-  for (int k = 0; k < NB; ++k)
-    if (freem_f[(long)i * NB + k]) x_f[(long)i * NB + k] = (T)((double)x_f[(long)i * NB + k] + acc[k]);
+  x_f[gid] = (T)((double)x_f[gid] + acc);
 }
 
-// x = omega Dinv b
+// x = omega Dinv b; one thread per dof
 template <int NB, typename T>
 __global__ void k_jac_first(int n_rows, const T* dinv, const T* b, double omega, T* x) {
-  int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n_rows) return;
-  double v[NB];
+  long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (gid >= (long)n_rows * NB) return;
+  long row = gid / NB;
+  int r = (int)(gid % NB);
+  double s = 0.0;
 #pragma unroll
-  for (int j = 0; j < NB; ++j) v[j] = (double)b[(long)row * NB + j];
-#pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    double s = 0.0;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) s += (double)dinv[(long)row * NB * NB + r * NB + j] * v[j];
-    x[(long)row * NB + r] = (T)(omega * s);
-  }
+  for (int j = 0; j < NB; ++j) s += (double)dinv[row * NB * NB + r * NB + j] * (double)b[row * NB + j];
+  x[gid] = (T)(omega * s);
 }
 
 // x = Minv b for the coarsest level's dense inverse: one wavefront per row, lanes
@@ -1378,7 +1368,6 @@ template <int DIM, int NB, typename T>
 static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
   const gm_mg_level& lv = cy.L[l];
   int n = lv.n_nodes;
-  int rb = gm_blocks(n, 256);
   const T* dinv = (const T*)lv.dinv;
   T* r = (T*)lv.r;
   bool coarsest = (l == cy.n_levels - 1);
@@ -1389,7 +1378,7 @@ static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
     return 0;
   }
   int sweeps = coarsest ? cy.coarse_sweeps : cy.nu_pre;
-  k_jac_first<NB, T><<<rb, 256, 0, st>>>(n, dinv, b, cy.omega, x);
+  k_jac_first<NB, T><<<gm_blocks((long)n * NB, 256), 256, 0, st>>>(n, dinv, b, cy.omega, x);
   GM_LAUNCH_CHECK();
   int rc = smooth<NB, T>(lv, l, b, x, sweeps - 1, cy.omega, st);
   if (rc) return rc;
@@ -1398,12 +1387,13 @@ static int vcycle(const Cycle& cy, int l, const T* b, T* x, hipStream_t st) {
   const int gamma = lv.gamma > 1 ? lv.gamma : 1;  // 1: V-cycle, 2: W-cycle at this level
   for (int g = 0; g < gamma; ++g) {
     level_op<NB, 1, T>(lv, l, x, b, nullptr, 0.0, r, st);  // r = b - A x
-    k_restrict<DIM, NB, T><<<gm_blocks(cv.n_nodes, 256), 256, 0, st>>>(dims_of(lv), dims_of(cv), cv.n_nodes,
-                                                                        lv.freem, r, (T*)cv.b);
+    k_restrict<DIM, NB, T><<<gm_blocks((long)cv.n_nodes * NB, 256), 256, 0, st>>>(
+        dims_of(lv), dims_of(cv), cv.n_nodes, lv.freem, r, (T*)cv.b);
     GM_LAUNCH_CHECK();
     rc = vcycle<DIM, NB, T>(cy, l + 1, (const T*)cv.b, (T*)cv.x, st);
     if (rc) return rc;
-    k_prolong_add<DIM, NB, T><<<rb, 256, 0, st>>>(dims_of(lv), dims_of(cv), n, lv.freem, (const T*)cv.x, x);
+    k_prolong_add<DIM, NB, T><<<gm_blocks((long)n * NB, 256), 256, 0, st>>>(dims_of(lv), dims_of(cv), n, lv.freem,
+                                                                             (const T*)cv.x, x);
   }
   return smooth<NB, T>(lv, l, b, x, cy.nu_post, cy.omega, st);
 }
