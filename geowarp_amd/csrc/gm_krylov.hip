@@ -781,24 +781,39 @@ __global__ void k_multidot_fin(int nchunk, const double* part, double* h) {
   if (threadIdx.x == 0) h[i] = sh[0];
 }
 
-// w -= sum_i V_i h_i   (h on the device)
+// w -= sum_i V_i h_i   (h on the device); one row per thread, four independent partial
+// sums over the basis vectors so that four loads are in flight per thread
 template <typename TV>
-__global__ void k_multiaxpy(long n, int k, const TV* V, long ldv, const double* h, TV* w) {
-  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
-    double acc = (double)w[r];
-    for (int i = 0; i < k; ++i) acc -= h[i] * (double)V[(long)i * ldv + r];
-    w[r] = (TV)acc;
+__global__ void __launch_bounds__(256) k_multiaxpy(long n, int k, const TV* V, long ldv, const double* h, TV* w) {
+  long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = 0;
+  for (; i + 3 < k; i += 4) {
+    a0 += h[i] * (double)V[(long)i * ldv + r];
+    a1 += h[i + 1] * (double)V[(long)(i + 1) * ldv + r];
+    a2 += h[i + 2] * (double)V[(long)(i + 2) * ldv + r];
+    a3 += h[i + 3] * (double)V[(long)(i + 3) * ldv + r];
   }
+  for (; i < k; ++i) a0 += h[i] * (double)V[(long)i * ldv + r];
+  w[r] = (TV)((double)w[r] - ((a0 + a1) + (a2 + a3)));
 }
 
-// out = sum_i V_i y_i   (y on the device)
+// out = sum_i V_i y_i   (y on the device); one row per thread, four partial sums
 template <typename TV>
-__global__ void k_combine(long n, int k, const TV* V, long ldv, const double* y, TV* out) {
-  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int i = 0; i < k; ++i) acc += y[i] * (double)V[(long)i * ldv + r];
-    out[r] = (TV)acc;
+__global__ void __launch_bounds__(256) k_combine(long n, int k, const TV* V, long ldv, const double* y, TV* out) {
+  long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = 0;
+  for (; i + 3 < k; i += 4) {
+    a0 += y[i] * (double)V[(long)i * ldv + r];
+    a1 += y[i + 1] * (double)V[(long)(i + 1) * ldv + r];
+    a2 += y[i + 2] * (double)V[(long)(i + 2) * ldv + r];
+    a3 += y[i + 3] * (double)V[(long)(i + 3) * ldv + r];
   }
+  for (; i < k; ++i) a0 += y[i] * (double)V[(long)i * ldv + r];
+  out[r] = (TV)((a0 + a1) + (a2 + a3));
 }
 
 template <typename TV>
@@ -872,7 +887,9 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
   long n = (long)n_rows * NB;
   long ldv = n;
   int vb = dot_blocks(n);
-  int nchunk = 64;
+  // multidot chunks per basis vector: up to 256 while few vectors exist (fills the chip), within
+  // the 64 (m + 1) partial sums of the workspace
+  auto chunks = [m](int cols) { return std::min(256, 64 * (m + 1) / cols); };
   GM_HIP(hipMemsetAsync(x, 0, sizeof(TV) * n, st));
   // ||b||
   k_dot2<TV><<<vb, RED_T, 0, st>>>(n, b, b, nullptr, nullptr, part);
@@ -934,9 +951,10 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
       const int nsl = 2 * (m + 2);
       k_dot2<TV><<<vb, RED_T, 0, st>>>(n, w, w, nullptr, nullptr, part);
       k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + nsl + 1, nullptr);
+      int nchunk = chunks(j + 1);
       k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, st>>>(n, j + 1, V, ldv, w, nchunk, part);
       k_multidot_fin<<<j + 1, RED_T, 0, st>>>(nchunk, part, dh);
-      k_multiaxpy<TV><<<vb, RED_T, 0, st>>>(n, j + 1, V, ldv, dh, w);
+      k_multiaxpy<TV><<<gm_blocks(n, 256), 256, 0, st>>>(n, j + 1, V, ldv, dh, w);
       k_dot2<TV><<<vb, RED_T, 0, st>>>(n, w, w, nullptr, nullptr, part);
       k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + nsl, nullptr);
       GM_LAUNCH_CHECK();
@@ -947,7 +965,7 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
       if (!(hh[nsl] >= 0.5 * hh[nsl + 1])) {  // also taken when a norm is NaN
         k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, st>>>(n, j + 1, V, ldv, w, nchunk, part);
         k_multidot_fin<<<j + 1, RED_T, 0, st>>>(nchunk, part, dh + (m + 2));
-        k_multiaxpy<TV><<<vb, RED_T, 0, st>>>(n, j + 1, V, ldv, dh + (m + 2), w);
+        k_multiaxpy<TV><<<gm_blocks(n, 256), 256, 0, st>>>(n, j + 1, V, ldv, dh + (m + 2), w);
         k_dot2<TV><<<vb, RED_T, 0, st>>>(n, w, w, nullptr, nullptr, part);
         k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + nsl, nullptr);
         GM_LAUNCH_CHECK();
@@ -995,11 +1013,11 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
     }
     GM_HIP(hipMemcpyAsync(dh, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
     if (Zs) {
-      k_combine<TV><<<vb, RED_T, 0, st>>>(n, j, Zs, ldv, dh, w);
+      k_combine<TV><<<gm_blocks(n, 256), 256, 0, st>>>(n, j, Zs, ldv, dh, w);
       k_add_into<TV><<<vb, RED_T, 0, st>>>(n, w, x);
       GM_LAUNCH_CHECK();
     } else {
-      k_combine<TV><<<vb, RED_T, 0, st>>>(n, j, V, ldv, dh, w);
+      k_combine<TV><<<gm_blocks(n, 256), 256, 0, st>>>(n, j, V, ldv, dh, w);
       GM_LAUNCH_CHECK();
       int prc = prec.apply_add(w, x, z, st);
       if (prc) return prc;
