@@ -764,13 +764,20 @@ extern "C" int gm_node_residual(const gm_grid* g, const gm_bins* b, const double
 // ================================================================== node Jacobian
 // One 64-lane wavefront per block row.  Lane l owns compressed slots l and l+64.
 template <int DIM, bool COUPLED, typename TD, typename TV>
-__global__ void __launch_bounds__(64) k_node_jacobian(gm_grid g, const double* x_s, const double* lp_s,
+__global__ void __launch_bounds__(64, 3) k_node_jacobian(gm_grid g, const double* x_s, const double* lp_s,
                                                       const int* cell_start, const uint8_t* ext_s, const TD* D,
                                                       const uint8_t* freem, const int* row_ptr,
                                                       const uint64_t* row_mask, TV* vals) {
   typedef Lay<DIM, COUPLED> L;
   const int NB = L::NB, NZ = L::NZ, NY = L::NY;
-  __shared__ double E[NB * NZ];
+  // the particles of a bin are taken JB at a time: their axis data, support flags and
+  // E = L_I D_p rows are formed in parallel, then every slot lane accumulates them in
+  // bin order (the order and arithmetic of one-particle-at-a-time processing)
+  constexpr int JB = 8;
+  __shared__ double E[JB][NB * NZ];
+  __shared__ Axis AX[JB][DIM];
+  __shared__ int INS[JB];
+  __shared__ uint8_t EXS[JB];
   int I = blockIdx.x;
   if (I >= g.n_nodes) return;
   int lane = threadIdx.x;
@@ -808,64 +815,99 @@ __global__ void __launch_bounds__(64) k_node_jacobian(gm_grid g, const double* x
 #pragma unroll
     for (int k = 0; k < NB * NB; ++k) acc[s2][k] = 0.0;
 
-  for_candidates<DIM>(g, ci, cell_start, ext_s, [&](int k, const int* o, const int* b, bool in) {
-    if (!in) return;  // uniform across the wavefront
-    Axis ax[DIM];
-    load_axes<DIM>(g, x_s, lp_s, k, ax);
-    double wI, GI[3], waI;
-    node_geom<DIM>(ax, o, wI, GI, waI);
-    const TD* Dp = D + (long)k * NY * NZ;
-    // E = L_I D_p  (NB x NZ), one entry per lane
-    if (lane < NB * NZ) {
-      int r = lane / NZ, z = lane % NZ;
-      double s = 0.0;
-      if (r < DIM) {
+  int ncell[3] = {g.nx, g.ny, g.nz};
+  const int oz_n = (DIM == 3) ? 3 : 1;
+  for (int oz = 0; oz < oz_n; ++oz)
+    for (int oy = 0; oy < 3; ++oy)
+      for (int ox = 0; ox < 3; ++ox) {
+        // base cell I - o: its particles have node I at stencil offset o (candidate walk order)
+        int o[3] = {ox, oy, oz};
+        int bc[3];
+        bool ok = true;
 #pragma unroll
-        for (int bb = 0; bb < DIM; ++bb) s += GI[bb] * (double)Dp[(bb * DIM + r) * NZ + z];
-        s += wI * (double)Dp[(L::Y_T + r) * NZ + z];
-      } else {
-        s = wI * (double)Dp[L::Y_Q * NZ + z];
-#pragma unroll
-        for (int bb = 0; bb < DIM; ++bb) s += GI[bb] * (double)Dp[(L::Y_M + bb) * NZ + z];
-        s += (wI - waI) * (double)Dp[L::Y_S * NZ + z];
-      }
-      E[lane] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s2 = 0; s2 < 2; ++s2) {
-      if (!has[s2]) continue;
-      int oJ[3];
-      bool inJ = true;
-#pragma unroll
-      for (int aa = 0; aa < DIM; ++aa) {
-        oJ[aa] = o[aa] + myoff[s2][aa];  // offset of J inside this particle's stencil
-        int e = ((ext_s[k] >> aa) & 1) ? 3 : 2;
-        inJ = inJ && (oJ[aa] >= 0) && (oJ[aa] < e);
-      }
-      if (!inJ) continue;
-      double wJ, GJ[3], waJ;
-      node_geom<DIM>(ax, oJ, wJ, GJ, waJ);
-#pragma unroll
-      for (int r = 0; r < NB; ++r) {
-#pragma unroll
-        for (int e = 0; e < DIM; ++e) {
-          double s = 0.0;
-#pragma unroll
-          for (int aa = 0; aa < DIM; ++aa) s += E[r * NZ + aa * DIM + e] * GJ[aa];
-          if (DIM == 3 && COUPLED && e == 2) s += E[r * NZ + L::Z_DU] * wJ;
-          acc[s2][r * NB + e] += s;
+        for (int a = 0; a < DIM; ++a) {
+          bc[a] = ci[a] - o[a];
+          if (bc[a] < 0 || bc[a] > ncell[a]) ok = false;
         }
-        if (COUPLED) {
-          double s = E[r * NZ + L::Z_P] * wJ + E[r * NZ + L::Z_PB] * waJ;
+        if (!ok) continue;
+        int bn = node_index<DIM>(g, bc);
+        int ks = cell_start[bn], ke = cell_start[bn + 1];
+        for (int k0 = ks; k0 < ke; k0 += JB) {
+          const int nj = min(JB, ke - k0);
+          if (lane < nj) {
+            int k = k0 + lane;
+            uint8_t ex = ext_s[k];
+            bool in = true;
 #pragma unroll
-          for (int aa = 0; aa < DIM; ++aa) s += E[r * NZ + L::Z_G + aa] * GJ[aa];
-          acc[s2][r * NB + DIM] += s;
+            for (int a = 0; a < DIM; ++a) in = in && (o[a] < (((ex >> a) & 1) ? 3 : 2));
+            INS[lane] = in ? 1 : 0;
+            EXS[lane] = ex;
+            Axis ax[DIM];
+            load_axes<DIM>(g, x_s, lp_s, k, ax);
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) AX[lane][a] = ax[a];
+          }
+          __syncthreads();
+          // E = L_I D_p  (NB x NZ) for each particle of the batch
+          for (int q = lane; q < nj * NB * NZ; q += 64) {
+            int pj = q / (NB * NZ), qq = q % (NB * NZ);
+            if (!INS[pj]) continue;
+            double wI, GI[3], waI;
+            node_geom<DIM>(AX[pj], o, wI, GI, waI);
+            const TD* Dp = D + (long)(k0 + pj) * NY * NZ;
+            int r = qq / NZ, z = qq % NZ;
+            double sE = 0.0;
+            if (r < DIM) {
+#pragma unroll
+              for (int bb = 0; bb < DIM; ++bb) sE += GI[bb] * (double)Dp[(bb * DIM + r) * NZ + z];
+              sE += wI * (double)Dp[(L::Y_T + r) * NZ + z];
+            } else {
+              sE = wI * (double)Dp[L::Y_Q * NZ + z];
+#pragma unroll
+              for (int bb = 0; bb < DIM; ++bb) sE += GI[bb] * (double)Dp[(L::Y_M + bb) * NZ + z];
+              sE += (wI - waI) * (double)Dp[L::Y_S * NZ + z];
+            }
+            E[pj][qq] = sE;
+          }
+          __syncthreads();
+          for (int pj = 0; pj < nj; ++pj) {
+            if (!INS[pj]) continue;  // uniform across the workgroup
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2) {
+              if (!has[s2]) continue;
+              int oJ[3];
+              bool inJ = true;
+#pragma unroll
+              for (int aa = 0; aa < DIM; ++aa) {
+                oJ[aa] = o[aa] + myoff[s2][aa];  // offset of J inside this particle's stencil
+                int e = ((EXS[pj] >> aa) & 1) ? 3 : 2;
+                inJ = inJ && (oJ[aa] >= 0) && (oJ[aa] < e);
+              }
+              if (!inJ) continue;
+              double wJ, GJ[3], waJ;
+              node_geom<DIM>(AX[pj], oJ, wJ, GJ, waJ);
+#pragma unroll
+              for (int r = 0; r < NB; ++r) {
+#pragma unroll
+                for (int e = 0; e < DIM; ++e) {
+                  double sv = 0.0;
+#pragma unroll
+                  for (int aa = 0; aa < DIM; ++aa) sv += E[pj][r * NZ + aa * DIM + e] * GJ[aa];
+                  if (DIM == 3 && COUPLED && e == 2) sv += E[pj][r * NZ + L::Z_DU] * wJ;
+                  acc[s2][r * NB + e] += sv;
+                }
+                if (COUPLED) {
+                  double sv = E[pj][r * NZ + L::Z_P] * wJ + E[pj][r * NZ + L::Z_PB] * waJ;
+#pragma unroll
+                  for (int aa = 0; aa < DIM; ++aa) sv += E[pj][r * NZ + L::Z_G + aa] * GJ[aa];
+                  acc[s2][r * NB + DIM] += sv;
+                }
+              }
+            }
+          }
+          __syncthreads();
         }
       }
-    }
-    __syncthreads();
-  });
   // masks, sign (the system matrix is -dR/du) and identity on fixed/inactive dofs
 #pragma unroll
   for (int s2 = 0; s2 < 2; ++s2) {
