@@ -22,6 +22,13 @@ region; algorithmic bytes per launch = nnzb*(b^2*s + 4) + 4*(n_nodes+1) + 2*n*s.
 cpu_baseline: the oracle (NumPy, one thread) on a bounded sample: one Newton
 iteration (residual + coloured-FD Jacobian + sparse direct solve) of the same
 workload restricted to 8 x 8 x 4 cells, scaled to the full particle count.
+
+Further blocks of the same JSON line (each skippable by its option): the same
+steps in the mixed form with Newton run to 1e-11 (field errors set by precision,
+not by the tolerance); c2_j2_bar, the 2D J2 bar of 256 x 1024 cells (FP64 and
+mixed); c3_consolidation, the 2D consolidation column against Terzaghi's series;
+adjoint, the batched 64-specimen inverse-analysis gradient with a central
+finite-difference check of one component.
 """
 import argparse
 import json
