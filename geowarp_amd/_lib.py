@@ -113,6 +113,7 @@ _SIGS = {
     'gm_bsr_transpose': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P]),
     'gm_mg_setup': (c_int, [c_int, c_int, c_int, c_int, _P, _P, _P]),
     'gm_mg_vcycle': (c_int, [c_int, c_int, c_int, c_int, _P, c_int, c_int, c_int, c_double, _P, _P, _P]),
+    'gm_mg_level_op': (c_int, [c_int, c_int, _P, c_int, c_int, _P, _P, c_double, _P, _P]),
     'gm_gmres_mg': (c_int, [c_int, c_int, c_int, c_int, _P, c_int, c_int, c_int, c_double, _P, _P, _P, _P, _P,
                             _P, _P, _P, c_int, c_double, c_int, _P, _P, _P]),
     'gm_spmv_autotune': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, c_int, _P, _P, _P]),

@@ -1515,6 +1515,44 @@ extern "C" int gm_mg_setup(int dim, int nb, int fp32, int n_levels, gm_mg_level*
   return -1;
 }
 
+// One smoother-side operation on one level (the fused kernel the V-cycle runs):
+// mode 1: out = b - A x; mode 2: out = x + omega Dinv (b - A x).
+extern "C" int gm_mg_level_op(int nb, int fp32, const gm_mg_level* level, int level_index, int mode, const void* x,
+                              const void* b, double omega, void* out, hipStream_t s) {
+  GM_REQUIRE(level && x && b && out && level->row_ptr && level->col_idx && level->vals,
+             "gm_mg_level_op: null argument");
+  GM_REQUIRE(nb >= 1 && nb <= 4, "gm_mg_level_op: nb must be 1..4");
+  GM_REQUIRE(mode == 1 || mode == 2, "gm_mg_level_op: mode must be 1 (residual) or 2 (Jacobi sweep)");
+  GM_REQUIRE(mode == 1 || level->dinv, "gm_mg_level_op: the Jacobi sweep needs the level's dinv");
+  GM_REQUIRE(!level->vals_lo || (fp32 && level->row_scale), "gm_mg_level_op: FP16 copy needs FP32 and scales");
+  GM_REQUIRE(x != out, "gm_mg_level_op: out must not alias x");
+#define GM_LOP(B)                                                                                              \
+  do {                                                                                                         \
+    if (fp32) {                                                                                                \
+      if (mode == 1)                                                                                           \
+        mg::level_op<B, 1, float>(*level, level_index, (const float*)x, (const float*)b, nullptr, omega,       \
+                                  (float*)out, s);                                                             \
+      else                                                                                                     \
+        mg::level_op<B, 2, float>(*level, level_index, (const float*)x, (const float*)b,                       \
+                                  (const float*)level->dinv, omega, (float*)out, s);                           \
+    } else {                                                                                                   \
+      if (mode == 1)                                                                                           \
+        mg::level_op<B, 1, double>(*level, level_index, (const double*)x, (const double*)b, nullptr, omega,    \
+                                   (double*)out, s);                                                           \
+      else                                                                                                     \
+        mg::level_op<B, 2, double>(*level, level_index, (const double*)x, (const double*)b,                    \
+                                   (const double*)level->dinv, omega, (double*)out, s);                        \
+    }                                                                                                          \
+  } while (0)
+  if (nb == 1) GM_LOP(1);
+  if (nb == 2) GM_LOP(2);
+  if (nb == 3) GM_LOP(3);
+  if (nb == 4) GM_LOP(4);
+#undef GM_LOP
+  GM_LAUNCH_CHECK();
+  return 0;
+}
+
 extern "C" int gm_mg_vcycle(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre,
                             int nu_post, int coarse_sweeps, double omega, const void* b, void* x, hipStream_t s) {
   int rc = mg_check(dim, nb, n_levels, levels, fp32);

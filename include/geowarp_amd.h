@@ -236,6 +236,13 @@ int gm_mg_setup(int dim, int nb, int fp32, int n_levels, gm_mg_level* levels, vo
 int gm_mg_vcycle(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre, int nu_post,
                  int coarse_sweeps, double omega, const void* b, void* x, hipStream_t stream);
 
+/* One fused smoother-side operation on a single level, as the V-cycle runs it: mode 1
+   out = b - A x, mode 2 (damped block-Jacobi sweep) out = x + omega Dinv (b - A x), from the
+   level's FP16 copy when it has one.  level_index 0 means the level matrix holds the solve
+   precision, > 0 FP64.  No reference counterpart (the reference has no multigrid). */
+int gm_mg_level_op(int nb, int fp32, const gm_mg_level* level, int level_index, int mode, const void* x,
+                   const void* b, double omega, void* out, hipStream_t stream);
+
 /* Flexible GMRES(m) on level 0's matrix, right-preconditioned by the V-cycle;
    arguments as gm_gmres (V: (m+1) * n vectors, Z: m * n preconditioned
    directions, z, w: n; part >= 64(m+1) and >= 2048 doubles; dh >= 3(m+2)

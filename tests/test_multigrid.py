@@ -226,3 +226,36 @@ def test_bicgstab_on_3d_up_system():
     r = np.linalg.norm(A @ x.cpu().numpy() - b) / np.linalg.norm(b)
     print('bicgstab its', its, 'relres', rel, 'scipy relres', r)
     assert rel <= 1e-10 and r < 1e-8
+
+
+@pytest.mark.gpu
+def test_level_op_matches_scipy():
+    """gm_mg_level_op on level 0 of an FP64 hierarchy: mode 1 is b - A x and mode 2 is
+    x + omega Dinv (b - A x) with Dinv the inverses of the 4x4 diagonal blocks
+    (NumPy), both to 1e-12 relative."""
+    import ctypes
+    from geowarp_amd import _lib as L
+    from geowarp_amd.solvers._engine import _stream
+    s, eng = _first_system('indent_nh')
+    mg = eng.multigrid(False)[0]
+    st = _stream()
+    mg.setup(st)
+    nb, n_nodes = eng.nb, eng.n_nodes
+    A = _bsr_interleaved(eng.row_ptr, eng.col_idx, eng.vals, n_nodes, nb)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=n_nodes * nb)
+    b = rng.normal(size=n_nodes * nb)
+    xd = torch.from_numpy(x).to(eng.dev)
+    bd = torch.from_numpy(b).to(eng.dev)
+    res = b - A @ x
+    Ad = A.toarray()
+    dinv = np.stack([np.linalg.inv(Ad[i * nb:(i + 1) * nb, i * nb:(i + 1) * nb]) for i in range(n_nodes)])
+    sweep = x + mg.omega * np.einsum('nij,nj->ni', dinv, res.reshape(n_nodes, nb)).reshape(-1)
+    for mode, ref in ((1, res), (2, sweep)):
+        out = torch.zeros_like(xd)
+        L.check(L.lib().gm_mg_level_op(nb, 0, ctypes.addressof(mg.levels[0]), 0, mode, L.ptr(xd), L.ptr(bd),
+                                       mg.omega, L.ptr(out), st), 'gm_mg_level_op')
+        torch.cuda.synchronize()
+        err = np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref)
+        print('mode', mode, 'rel err', err)
+        assert err < 1e-12
