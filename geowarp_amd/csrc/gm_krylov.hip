@@ -195,7 +195,35 @@ __device__ __forceinline__ void row_acc_lo(int row, int lane, int n_rows, const 
   for (int q = 0; q < NB * NB; ++q) a[q] = 0.0f;
   if (row < n_rows) {
     int s = row_ptr[row], e = row_ptr[row + 1];
-    for (int k = s + lane; k < e; k += G) {
+    int k = s + lane;
+    // two blocks per step: both blocks' loads are issued before either is accumulated (the
+    // accumulation order is the one-block-at-a-time order)
+    for (; k + G < e; k += 2 * G) {
+      int c0 = col_idx[k], c1 = col_idx[k + G];
+      const gm_half* b0 = lo + (long)k * NB * NB;
+      const gm_half* b1 = lo + (long)(k + G) * NB * NB;
+      gm_half h0[NB * NB], h1[NB * NB];
+      float x0[NB], x1[NB];
+#pragma unroll
+      for (int q = 0; q < NB * NB; ++q) {
+        h0[q] = b0[q];
+        h1[q] = b1[q];
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        x0[j] = x[(long)c0 * NB + j];
+        x1[j] = x[(long)c1 * NB + j];
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) a[r * NB + j] += (float)h0[r * NB + j] * x0[j];
+#pragma unroll
+      for (int r = 0; r < NB; ++r)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) a[r * NB + j] += (float)h1[r * NB + j] * x1[j];
+    }
+    for (; k < e; k += G) {
       int c = col_idx[k];
       const gm_half* blk = lo + (long)k * NB * NB;
       float xv[NB];
