@@ -771,8 +771,8 @@ extern "C" int gm_fix_empty_diag(int n_rows, const int* row_ptr, const int* col_
 
 // ================================================================== restarted GMRES(m)
 // Right-preconditioned GMRES(m) (Saad & Schultz 1986) with block-Jacobi M:
-// Arnoldi by classical Gram-Schmidt applied twice (CGS2) on device-resident
-// basis vectors; the small Hessenberg least-squares problem (Givens rotations)
+// Arnoldi by classical Gram-Schmidt with selective reorthogonalisation on
+// device-resident basis vectors; the small Hessenberg least-squares problem (Givens rotations)
 // is solved on the host, which reads one column of H per iteration.
 // x <- x + M^{-1} V y at the end of every cycle; restart from r = b - A x.
 
@@ -902,8 +902,9 @@ struct JacobiPrec {
   }
 };
 
-// Restarted GMRES(m), right preconditioned (prec.apply / prec.apply_add), CGS2
-// Arnoldi with device dot products, Givens rotations on the host.
+// Restarted GMRES(m), right preconditioned (prec.apply / prec.apply_add), classical
+// Gram-Schmidt Arnoldi (second pass when needed) with device dot products, Givens
+// rotations on the host.
 // With Zs (m vectors) the method is flexible GMRES (Saad 1993): the
 // preconditioned directions z_j = M^-1 v_j are kept and x += Z y, so the
 // residual estimate stays exact when M^-1 is not applied identically each time

@@ -123,7 +123,8 @@ int gm_bicgstab(int nb, int fp32, int n_rows, const int* row_ptr, const int* col
                 const void* b, void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int* iters,
                 double* relres, hipStream_t stream);
 
-/* Restarted right-preconditioned GMRES(m) with block-Jacobi and CGS2 Arnoldi on
+/* Restarted right-preconditioned GMRES(m) with block-Jacobi and classical Gram-Schmidt
+   Arnoldi (reorthogonalised when a pass removes more than half of the vector's norm) on
    the same matrices (the robust alternative to BiCGStab, no breakdown).  V:
    (m+1)*n_rows*nb entries of the solve precision, z and w: n_rows*nb each,
    dinv: n_rows*nb*nb, part: >= max(64*(m+1), 2048) doubles, dh: >= 3*(m+2)
