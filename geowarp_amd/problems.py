@@ -120,6 +120,88 @@ def bar_j2(n_cells_x=256, n_cells_y=1024, dx=1.953125e-3, total_steps=50, tol=1e
     )
 
 
+def make_cantilever_bc(end_y, d0, dx):
+    """cantilever_beam_cpGIMP.py:98-113: x fixed on node columns ix <= 1; y fixed
+    on those columns at the node row int((end_y - d0/2 + dx/2)/dx) only."""
+
+    def set_boundary_dofs(boundary, n_grid_x, n_nodes):
+        nx1 = n_grid_x + 1
+        bx = _view2d(boundary, 0, n_nodes, nx1)
+        by = _view2d(boundary, 1, n_nodes, nx1)
+        bx[:, :2] = True
+        by[int((end_y - d0 / 2.0 + 0.5 * dx) / dx), :2] = True
+
+    return set_boundary_dofs
+
+
+def cantilever(n_grid=60, max_len=15.0, l0=10.0, d0=1.0, PPD=6, E=12000.0, nu=0.2, tip_load=-50.0, total_steps=50,
+               tol=1e-8, precision='fp64'):
+    """Plane-strain cantilever (cantilever_beam_cpGIMP.py:11-167): beam l0 x d0
+    clamped on the left (x on node columns 0-1, y on the mid-height node row of
+    those columns), Hencky elasticity, a point load on the particles of the free
+    end at mid-height (set_external_force_flag, :116-133) ramped over the load
+    steps."""
+    dx = max_len / n_grid
+    sx = dx
+    g = dict(n_grid_x=n_grid, n_grid_y=n_grid, max_x=max_len, max_y=max_len, dx=dx, dy=dx, start_x=sx,
+             end_x=sx + l0, start_y=l0 - d0, end_y=l0, domain_min_x=sx, domain_max_x=max_len - dx, domain_min_y=dx,
+             domain_max_y=max_len - dx)
+    npart = int(round(l0 / dx)) * int(round(d0 / dx)) * PPD * PPD
+
+    def external_flags(x, ext_flag):
+        tip = x[:, 0] > sx + l0 - dx / PPD
+        mid = (x[:, 1] > l0 - d0 / 2.0 - dx / PPD) & (x[:, 1] < l0 - d0 / 2.0 + dx / PPD)
+        ext_flag[tip & mid] = True
+
+    return dict(
+        solver='quasi_static_solver_2d',
+        background_grid_dict=g,
+        particles_dict=dict(n_particles=npart, PPD=PPD, p_vol=(dx / PPD) ** 2, p_rho=1.0),
+        iteration_dict=dict(n_iter=10, tol=tol, tol_mode='relative', precision=precision),
+        material_dict=dict(youngs_modulus=E, poisson_ratio=nu, material_name='Hencky elasticity'),
+        load_dict=dict(gravity_mag=0.0, traction_value_x=0.0, traction_value_y=0.0, point_load_value_x=0.0,
+                       point_load_value_y=tip_load),
+        set_boundary_dofs=make_cantilever_bc(l0, d0, dx),
+        external_flags=external_flags,
+        total_steps=total_steps,
+    )
+
+
+def run_load_steps(solver, cfg, n_steps=None, first=0, vtk_pattern=None, fields=('stress_xx', 'stress_yy'),
+                   on_step=None):
+    """The drivers' load loop (e.g. bar_compaction_cpGIMP.py:128-137,
+    terzaghi_cpGIMP.py:180-189): advance_one_step(step, total) for each step and,
+    with vtk_pattern (e.g. 'out/particles_%d.vtk'), one legacy-VTK particle file
+    per step (and one for the initial state) holding the named point data
+    (stress_xx, stress_yy, pressure).  on_step(step, solver) runs after each
+    step.  Returns the Newton iteration counts."""
+    from . import vtk
+
+    total = cfg['total_steps']
+    n = total - first if n_steps is None else n_steps
+
+    def dump(k):
+        if vtk_pattern is None:
+            return
+        sig = solver.particle_Cauchy_stress_array
+        avail = {'stress_xx': lambda: sig[:, 0, 0], 'stress_yy': lambda: sig[:, 1, 1],
+                 'stress_zz': lambda: sig[:, 2, 2],
+                 'pressure': lambda: solver.particle_pressure_array}
+        vtk.Mesh(points=solver.x_particles, cells=[],
+                 point_data={f: avail[f]() for f in fields if f in avail}).write(vtk_pattern % k)
+
+    if first == 0:
+        dump(0)
+    iters = []
+    for step in range(first, first + n):
+        solver.advance_one_step(step, total)
+        iters.append(solver.newton_iterations)
+        dump(step + 1)
+        if on_step is not None:
+            on_step(step, solver)
+    return iters
+
+
 def make_column_bc():
     """Terzaghi column: roller walls (x fixed at node_ix <= 1 or >= 2 for a one-cell
     column -> at the two wall lines), fixed base in y (terzaghi_cpGIMP.py:104-119)."""

@@ -19,7 +19,7 @@ sys.path.insert(0, os.path.abspath(os.path.join(HERE, '..')))
 import oracle.solvers as osv  # noqa: E402
 import harness as H  # noqa: E402
 
-STEPS = {'bar_nh': 3, 'bar_hencky': 2, 'bar_j2': 3, 'bar_dp': 3, 'bar_j2_platen': 4, 'terzaghi': 2, 'indent_nh': 2, 'indent_dp': 2}
+STEPS = {'bar_nh': 3, 'bar_hencky': 2, 'bar_j2': 3, 'bar_dp': 3, 'bar_j2_platen': 4, 'terzaghi': 2, 'indent_nh': 2, 'indent_dp': 2, 'cantilever': 3}
 
 
 def gen(name):

@@ -17,6 +17,7 @@ MINI = {
     'terzaghi': lambda: P.terzaghi(n_cells_x=2, n_cells_y=16, width=0.125, total_steps=10, dt=1.0),
     'indent_nh': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.05, footing=0.5,
                                     material='Neo-Hookean', tol=1e-10),
+    'cantilever': lambda: P.cantilever(PPD=2, total_steps=50, tol=1e-10),
     'indent_dp': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.01, footing=0.5,
                                     material='Drucker-Prager', tol=1e-10, n_iter=20),
 }
@@ -49,6 +50,8 @@ def make(module, cfg, **kw):
     if cfg['solver'] == 'quasi_static_solver_2d':
         flags = s.boundary if is_oracle else s.dofStruct.boundary_flag_array
         cfg['set_boundary_dofs'](flags, s.n_grid_x, s.n_nodes)
+    if 'external_flags' in cfg:
+        cfg['external_flags'](s.x_particles, s.external_flag if is_oracle else s.particle_external_flag_array)
     if 'particle_flags' in cfg:
         if is_oracle:
             tf = getattr(s, 'traction_flag', np.zeros(s.n_particles, dtype=bool))

@@ -14,7 +14,7 @@ import pytest
 import harness as H
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
-CASES = ['bar_nh', 'bar_hencky', 'bar_j2', 'bar_dp', 'bar_j2_platen', 'terzaghi', 'indent_nh']
+CASES = ['bar_nh', 'bar_hencky', 'bar_j2', 'bar_dp', 'bar_j2_platen', 'terzaghi', 'indent_nh', 'cantilever']
 FIELD_TOL = 1e-8
 
 
