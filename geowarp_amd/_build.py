@@ -9,7 +9,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libgeowarp_amd.so')
-SOURCES = ['gm_mpm.hip', 'gm_krylov.hip', 'gm_adjoint.hip']
+SOURCES = sorted(f for f in os.listdir(CSRC) if f.endswith('.hip'))  # every HIP source of csrc/
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-Wno-unused-result']

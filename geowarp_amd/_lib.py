@@ -100,6 +100,7 @@ _SIGS = {
     'gm_bsr_spmv': (c_int, [c_int, c_int, c_int, c_int, _P, _P, _P, _P, _P, _P]),
     'gm_norm2': (c_int, [c_long, _P, _P, _P, _P]),
     'gm_sum': (c_int, [c_long, _P, _P, _P, _P]),
+    'gm_norm2_noalloc': (c_int, [c_long, _P, _P, _P]),
     'gm_axpy': (c_int, [c_long, c_double, _P, c_int, _P, _P]),
     'gm_residual_refine': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
     'gm_d2f': (c_int, [c_long, _P, _P, _P]),

@@ -148,6 +148,14 @@ int gm_residual_refine(int nb, int mat_fp32, int n_rows, const int* row_ptr, con
 int gm_norm2(long n, const double* v, double* part, double* out, hipStream_t stream);
 int gm_sum(long n, const double* v, double* part, double* out, hipStream_t stream);
 
+/* gm_norm2 without a workspace argument: no counterpart in the reference; for
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops).  The partial sums live in a static 16 KiB device array
+   of the library (one per GPU, kept until the process ends); the result is
+   bitwise that of gm_norm2.  0 <= n <= 2^40, else -1 and gm_last_error().
+   Calls that overlap on one GPU (unordered streams) race on that array. */
+int gm_norm2_noalloc(long n, const double* v, double* out, hipStream_t stream);
+
 /* y += a*x (x FP64 or FP32): from_increment_to_solution (mpm_utils.py:1343-1348). */
 int gm_axpy(long n, double a, const void* x, int x_fp32, double* y, hipStream_t stream);
 
