@@ -30,6 +30,11 @@ class gm_pattern(ctypes.Structure):
     _fields_ = [('row_ptr', c_void_p), ('col_idx', c_void_p), ('row_mask', c_void_p), ('nnzb', c_int)]
 
 
+class gm_norsand_params(ctypes.Structure):
+    _fields_ = [(k, c_double) for k in ('kappa', 'tilde_lambda', 'M', 'N', 'beta', 'vc0', 'p0', 'v0', 'h', 'mu',
+                                        'ev0', 'tol')]
+
+
 class gm_stage_args(ctypes.Structure):
     _fields_ = [('n_particles', c_int), ('coupled', c_int), ('want_tangent', c_int), ('tangent_fp32', c_int),
                 ('x', c_void_p), ('lp', c_void_p), ('F_old', c_void_p), ('F_new', c_void_p), ('b_old', c_void_p),
@@ -43,12 +48,9 @@ class gm_stage_args(ctypes.Structure):
                 ('dt', c_double), ('PPD', c_double),
                 ('gravity', c_double * 3), ('point_load', c_double * 3), ('traction', c_double * 3),
                 ('footing', c_double * 4), ('footing_height', c_double), ('pen_par', c_double), ('penalty', c_int),
-                ('p_phi', c_void_p), ('p_coh', c_void_p)]
+                ('p_phi', c_void_p), ('p_coh', c_void_p), ('ns', gm_norsand_params)]
 
 
-class gm_norsand_params(ctypes.Structure):
-    _fields_ = [(k, c_double) for k in ('kappa', 'tilde_lambda', 'M', 'N', 'beta', 'vc0', 'p0', 'v0', 'h', 'mu',
-                                        'ev0', 'tol')]
 
 
 class gm_local_args(ctypes.Structure):
@@ -115,8 +117,11 @@ _SIGS = {
     'gm_mg_setup': (c_int, [c_int, c_int, c_int, c_int, _P, _P, _P]),
     'gm_mg_vcycle': (c_int, [c_int, c_int, c_int, c_int, _P, c_int, c_int, c_int, c_double, _P, _P, _P]),
     'gm_mg_level_op': (c_int, [c_int, c_int, _P, c_int, c_int, _P, _P, c_double, _P, _P]),
-    'gm_gmres_mg': (c_int, [c_int, c_int, c_int, c_int, _P, c_int, c_int, c_int, c_double, _P, _P, _P, _P, _P,
-                            _P, _P, _P, c_int, c_double, c_int, _P, _P, _P]),
+    'gm_gmres_mg': (c_int, [c_int, c_int, c_int, c_int, _P, _P, c_int, c_int, c_int, c_double, _P, _P, _P, _P,
+                            _P, _P, _P, _P, c_int, c_double, c_int, _P, _P, _P]),
+    'gm_penalty_pc': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern), _P, _P,
+                              c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_double), c_double,
+                              c_double, _P]),
     'gm_spmv_autotune': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, c_int, _P, _P, _P]),
     'gm_row_scale': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P]),
     'gm_up_setup': (c_int, [c_int, c_int, c_int, c_int, c_int, _P, _P, _P, _P, c_int, _P, _P, _P, _P]),

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gm_types.h"
+
 #define GM_DEV __device__ __forceinline__
 
 namespace gm {
@@ -224,7 +226,7 @@ GM_DEV void sym_eig3(const M3& Ain, double* lam, M3& V) {
 }
 
 // ------------------------------------------------------------------ materials
-enum { MAT_HENCKY = 0, MAT_J2 = 1, MAT_NEOHOOKEAN = 2, MAT_DP = 3 };
+enum { MAT_HENCKY = 0, MAT_J2 = 1, MAT_NEOHOOKEAN = 2, MAT_DP = 3, MAT_NORSAND = 4 };
 
 struct MatParams {
   int kind;
@@ -232,7 +234,173 @@ struct MatParams {
   double lam, mu, E;
   double kappa, H;             // J2
   double phi, psi, coh, shape;  // DP (degrees, kPa)
+  gm_norsand_params ns;        // NorSand
+  double lnJ;                  // NorSand: total volumetric strain ln J (specific volume v0 exp(ln J))
 };
+
+// ---- NorSand (triaxial_NorSand.py:155-400)
+GM_DEV double ns_eta(double M, double N, double P, double pi) {
+  if (fabs(N) < 1e-10) return M * (1.0 + log(pi / P));
+  return (M / N) * (1.0 - (1.0 - N) * pow(P / pi, N / (1.0 - N)));
+}
+GM_DEV double ns_deta_dP(double M, double N, double P, double pi) {
+  if (fabs(N) < 1e-10) return -M / P;
+  return -M * pow(P / pi, (2.0 * N - 1.0) / (1.0 - N)) / pi;
+}
+GM_DEV double ns_deta_dpi(double M, double N, double P, double pi) {
+  if (fabs(N) < 1e-10) return M / pi;
+  return M * pow(P / pi, (2.0 * N - 1.0) / (1.0 - N)) * P / (pi * pi);
+}
+GM_DEV double ns_eps_s(const double* e) {
+  double a = e[0] - e[1], b = e[1] - e[2], c = e[0] - e[2];
+  return sqrt(2.0 / 9.0 * (a * a + b * b + c * c));
+}
+
+// Local residual R and Jacobian dR/dx of the NorSand return at x = (ev, es, dl, pi) with
+// P = P(ev) (triaxial_NorSand.py:338-396); dR_dv (optional): dR/d(specific volume), nonzero
+// in the hardening row through pi*(psi_i).
+GM_DEV void ns_local(const gm_norsand_params& q, double erv, double ers, double dl, double pi, double ev_tr,
+                     double es_tr, double pi_old, double new_v, double r[4], double J[4][4], double* dR_dv) {
+  const double M = q.M, N = q.N, alpha_bar = -3.5 / q.beta;
+  const double Pt = q.p0 * exp(-(erv - q.ev0) / q.kappa);
+  const double dEdP = ns_deta_dP(M, N, Pt, pi), dEdPi = ns_deta_dpi(M, N, Pt, pi), eta = ns_eta(M, N, Pt, pi);
+  const double dFdP = (eta - M) / (1.0 - N);
+  const double psi_i = new_v - q.vc0 + q.tilde_lambda * log(-pi);
+  // pi*: the yield-test tolerance picks the branch for pi*, 1e-10 for its derivatives
+  // (triaxial_NorSand.py:356-362 and the derivative helpers)
+  const double pstar = fabs(N) < q.tol ? Pt * exp(alpha_bar * psi_i / M)
+                                       : Pt * pow(1.0 - alpha_bar * psi_i * N / M, (N - 1.0) / N);
+  const double dPs_dP = fabs(N) < 1e-10 ? exp(alpha_bar * psi_i / M)
+                                        : pow(1.0 - alpha_bar * psi_i * N / M, (N - 1.0) / N);
+  // d pi* / d psi_i
+  const double dPs_dpsi = fabs(N) < 1e-10 ? Pt * exp(alpha_bar * psi_i / M) * alpha_bar / M
+                                          : Pt * (N - 1.0) / N *
+                                                pow(1.0 / (1.0 - alpha_bar * psi_i * N / M), 1.0 / N) *
+                                                (-alpha_bar * N / M);
+  const double Qt = 3.0 * q.mu * ers;
+  r[0] = erv - ev_tr + dl * q.beta * dFdP;
+  r[1] = ers - es_tr + dl;
+  r[2] = Qt + eta * Pt;
+  r[3] = pi - pi_old + q.h * (pi - pstar) * dl;
+  const double dPdev = Pt / (-q.kappa);
+  J[0][0] = 1.0 + dl * q.beta * dEdP * dPdev / (1.0 - N);
+  J[0][1] = 0.0;
+  J[0][2] = q.beta * dFdP;
+  J[0][3] = dl * q.beta * dEdPi / (1.0 - N);
+  J[1][0] = 0.0;
+  J[1][1] = 1.0;
+  J[1][2] = 1.0;
+  J[1][3] = 0.0;
+  J[2][0] = (dEdP * Pt + eta) * dPdev;
+  J[2][1] = 3.0 * q.mu;
+  J[2][2] = 0.0;
+  J[2][3] = Pt * dEdPi;
+  J[3][0] = -q.h * dl * dPs_dP * dPdev;
+  J[3][1] = 0.0;
+  J[3][2] = q.h * (pi - pstar);
+  J[3][3] = 1.0 + q.h * dl * (1.0 - dPs_dpsi * q.tilde_lambda / pi);
+  if (dR_dv) {
+    dR_dv[0] = dR_dv[1] = dR_dv[2] = 0.0;
+    dR_dv[3] = -q.h * dl * dPs_dpsi;
+  }
+}
+
+GM_DEV void solve4(double M[4][4], double* r);
+
+// NorSand return of principal trial strains et with total volumetric strain ev_total and image
+// pressure pi_old: the reference's tension branch (zero strain when tr et > tol), yield test and
+// fixed 15-iteration local Newton.  Optionally the consistent tangent d e / d et (dE, 3x3,
+// row-major) and d e / d ln J (dEJ, 3) from the local system at the returned state (implicit
+// function theorem), and the last iterate's local residuals (res_last).
+GM_DEV void return_norsand(const double* et, double ev_total, double pi_old, const gm_norsand_params& q, double* e,
+                           double& pi_new, double* res_last, double* dE, double* dEJ) {
+  const double ev = et[0] + et[1] + et[2];
+  const double es = ns_eps_s(et);
+  double nd[3] = {0.0, 0.0, 0.0};
+  if (es > q.tol) {
+    const double s23 = sqrt(2.0 / 3.0);
+    for (int i = 0; i < 3; ++i) nd[i] = s23 * (et[i] - ev / 3.0) / es;
+  }
+  const double new_v = exp(ev_total) * q.v0;
+  double Pt = q.p0 * exp(-(ev - q.ev0) / q.kappa);
+  double Qt = 3.0 * q.mu * es;
+  e[0] = et[0];
+  e[1] = et[1];
+  e[2] = et[2];
+  pi_new = pi_old;
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  if (dE)
+    for (int i = 0; i < 9; ++i) dE[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  if (dEJ) dEJ[0] = dEJ[1] = dEJ[2] = 0.0;
+  if (ev > q.tol) {
+    e[0] = e[1] = e[2] = 0.0;
+    if (dE)
+      for (int i = 0; i < 9; ++i) dE[i] = 0.0;
+  } else if (Qt + ns_eta(q.M, q.N, Pt, pi_old) * Pt >= q.tol) {
+    double dl = 0.0, pi = pi_old;
+    const double s32 = sqrt(1.5);
+    for (int it = 0; it < 15; ++it) {
+      const double erv = e[0] + e[1] + e[2];
+      const double ers = ns_eps_s(e);
+      double J[4][4];
+      ns_local(q, erv, ers, dl, pi, ev, es, pi_old, new_v, r, J, nullptr);
+      double x[4] = {r[0], r[1], r[2], r[3]};
+      solve4(J, x);
+      const double nev = erv - x[0];
+      const double nes = ers - x[1];
+      dl -= x[2];
+      pi -= x[3];
+      for (int i = 0; i < 3; ++i) e[i] = nev / 3.0 + nes * nd[i] * s32;
+    }
+    pi_new = pi;
+    if (dE || dEJ) {
+      // d(ev, es, dl, pi) = J^-1 (d ev_tr e0 + d es_tr e1 - dR/dv dv) at the returned state
+      const double erv = e[0] + e[1] + e[2];
+      const double ers = ns_eps_s(e);
+      double J[4][4], rr[4], dRv[4];
+      ns_local(q, erv, ers, dl, pi, ev, es, pi_old, new_v, rr, J, dRv);
+      double X[3][4];
+      for (int c = 0; c < 3; ++c) {
+        double Jc[4][4];
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j) Jc[i][j] = J[i][j];
+        double b[4] = {c == 0 ? 1.0 : 0.0, c == 1 ? 1.0 : 0.0, 0.0, 0.0};
+        if (c == 2)
+          for (int i = 0; i < 4; ++i) b[i] = -dRv[i];
+        solve4(Jc, b);
+        for (int i = 0; i < 4; ++i) X[c][i] = b[i];
+      }
+      // e_i = ev/3 + sqrt(3/2) es nd_i, nd_i = sqrt(2/3)(et_i - ev_tr/3)/es_tr
+      if (dE) {
+        const double s23 = sqrt(2.0 / 3.0);
+        for (int j = 0; j < 3; ++j) {
+          const double des_tr = s23 * nd[j];  // d es_tr / d et_j
+          const double dev_j = X[0][0] + X[1][0] * des_tr;
+          const double des_j = X[0][1] + X[1][1] * des_tr;
+          for (int i = 0; i < 3; ++i) {
+            double dnd = (es > q.tol) ? s23 / es * ((i == j ? 1.0 : 0.0) - 1.0 / 3.0 - nd[i] * nd[j]) : 0.0;
+            dE[3 * i + j] = dev_j / 3.0 + s32 * (des_j * nd[i] + ers * dnd);
+          }
+        }
+      }
+      if (dEJ)
+        for (int i = 0; i < 3; ++i) dEJ[i] = (X[2][0] / 3.0 + s32 * X[2][1] * nd[i]) * new_v;
+    }
+  }
+  if (res_last)
+    for (int i = 0; i < 4; ++i) res_last[i] = r[i];
+}
+
+// Principal Kirchhoff stress of NorSand's pressure-dependent hyperelasticity
+// (triaxial_NorSand.py:455-461) and its derivative C = d tau / d e.
+GM_DEV void ns_tau(const double* e, const gm_norsand_params& q, double* t, double* C) {
+  const double tr = e[0] + e[1] + e[2];
+  const double P = q.p0 * exp(-(tr - q.ev0) / q.kappa);
+  for (int i = 0; i < 3; ++i) t[i] = P + 2.0 * q.mu * (e[i] - tr / 3.0);
+  if (C)
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) C[3 * i + k] = -P / q.kappa + 2.0 * q.mu * ((i == k ? 1.0 : 0.0) - 1.0 / 3.0);
+}
 
 // Principal return maps: eps_tr -> eps (3), d eps / d eps_tr (3x3), alpha.
 GM_DEV void tau_principal(const double* e, double lam, double mu, double* t) {
@@ -416,18 +584,23 @@ GM_DEV void return_dp(const double* etr, const MatParams& m, double* e, double* 
   for (int i = 0; i < 9; ++i) dE[i] = (i % 4 == 0) ? 1.0 : 0.0;
   if (y < 1e-10) return;
   double K = m.lam + 2.0 * m.mu / 3.0;
-  // a hyperbolic surface (shape factor > 0) has no apex: always return to it
-  bool smooth = (m.dp_apex == 1 && m.shape > 0.0);
-  bool tip = (P >= 0.0) && !smooth;
-  if (m.dp_apex == 1 && !tip && !smooth) {
-    // the cone return would reverse the deviator (closed form, shape factor 0)
+  bool tip;
+  if (m.dp_apex == 0) {
+    tip = (P >= 0.0);  // the reference: no return in tension (triaxial_DruckerPrager.py:345-349)
+  } else if (m.shape > 0.0) {
+    // hyperbolic surface: its tip (Q = 0) sits at P_tip = A (1 - a) / B; a trial state at or
+    // beyond it in tension has no return along the flow direction when the flow is deviatoric
+    // (dilation 0), and is cut off to the tip (tension cut-off)
+    tip = (P >= Af * (1.0 - m.shape) / Bf);
+  } else {
+    // cone: the closed-form cone return would reverse the deviator -> apex
     double Sn = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
     double dl_cone = y / (2.0 * m.mu + Bf * K * Bg);
     tip = (2.0 * m.mu * dl_cone >= Sn);
   }
   if (tip) {
     if (m.dp_apex == 1) {
-      double ev = (Af / Bf) / K;
+      double ev = (Af * (1.0 - m.shape) / Bf) / K;
       e[0] = e[1] = e[2] = ev / 3.0;
 #pragma unroll
       for (int i = 0; i < 9; ++i) dE[i] = 0.0;
@@ -470,6 +643,7 @@ struct SpectralState {
   double lam[3];  // eigenvalues of b_tr
   double tau[3];  // principal Kirchhoff
   double T[9];    // d tau_i / d eps_tr_k
+  double TJ[3];   // d tau_i / d ln J (NorSand: the specific volume enters the return)
 };
 
 GM_DEV void hencky_family(const M3& btr, const MatParams& m, double alpha_old, SpectralState& ss, M3& tau,
@@ -479,28 +653,49 @@ GM_DEV void hencky_family(const M3& btr, const MatParams& m, double alpha_old, S
 #pragma unroll
   for (int i = 0; i < 3; ++i) etr[i] = 0.5 * log(fmax(ss.lam[i], 1e-8));
   alpha_new = alpha_old;
-  if (m.kind == MAT_J2) {
-    return_j2(etr, m, alpha_old, e, dE, alpha_new);
-  } else if (m.kind == MAT_DP) {
-    return_dp(etr, m, e, dE);
-  } else {
-    e[0] = etr[0];
-    e[1] = etr[1];
-    e[2] = etr[2];
+  ss.TJ[0] = ss.TJ[1] = ss.TJ[2] = 0.0;
+  if (m.kind == MAT_NORSAND) {
+    // alpha carries the image pressure pi; elasticity is NorSand's own
+    double dEJ[3], C[9];
+    return_norsand(etr, m.lnJ, alpha_old, m.ns, e, alpha_new, nullptr, dE, dEJ);
+    ns_tau(e, m.ns, ss.tau, C);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) dE[i] = (i % 4 == 0) ? 1.0 : 0.0;
-  }
-  tau_principal(e, m.lam, m.mu, ss.tau);
-  // T = a_el * dE
+    for (int i = 0; i < 3; ++i) {
+      double sj = 0.0;
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) {
+        double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double s = 0.0;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) s += (m.lam + (i == j ? 2.0 * m.mu : 0.0)) * dE[3 * j + k];
-      ss.T[3 * i + k] = s;
+        for (int j = 0; j < 3; ++j) s += C[3 * i + j] * dE[3 * j + k];
+        ss.T[3 * i + k] = s;
+        sj += C[3 * i + k] * dEJ[k];
+      }
+      ss.TJ[i] = sj;
     }
+  } else {
+    if (m.kind == MAT_J2) {
+      return_j2(etr, m, alpha_old, e, dE, alpha_new);
+    } else if (m.kind == MAT_DP) {
+      return_dp(etr, m, e, dE);
+    } else {
+      e[0] = etr[0];
+      e[1] = etr[1];
+      e[2] = etr[2];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) dE[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    }
+    tau_principal(e, m.lam, m.mu, ss.tau);
+    // T = a_el * dE
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) s += (m.lam + (i == j ? 2.0 * m.mu : 0.0)) * dE[3 * j + k];
+        ss.T[3 * i + k] = s;
+      }
+  }
   double ex[3] = {exp(2.0 * e[0]), exp(2.0 * e[1]), exp(2.0 * e[2])};
 #pragma unroll
   for (int i = 0; i < 3; ++i)

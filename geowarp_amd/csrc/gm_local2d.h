@@ -214,16 +214,19 @@ GM_DD void l_return_dp(const T* etr, T lam, T mu, T phi, T psi, T coh, double sh
   for (int i = 0; i < 3; ++i) e[i] = etr[i];
   if (val(y) < 1e-10) return;
   T K = lam + 2.0 * mu / 3.0;
-  bool smooth = (apex == 1 && shape > 0.0);
-  bool tip = (val(P) >= 0.0) && !smooth;
-  if (apex == 1 && !tip && !smooth) {
+  bool tip;
+  if (apex == 0) {
+    tip = (val(P) >= 0.0);  // the reference: no return in tension
+  } else if (shape > 0.0) {
+    tip = (val(P) >= val(Af * (1.0 - shape) / Bf));  // at or beyond the hyperbola's tip: cut off
+  } else {
     T Sn = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
     T dl_cone = y / (2.0 * mu + Bf * K * Bg);
     tip = (val(2.0 * mu * dl_cone) >= val(Sn));
   }
   if (tip) {
     if (apex == 1) {
-      T ev = (Af / Bf) / K;
+      T ev = (Af * (1.0 - shape) / Bf) / K;
       for (int i = 0; i < 3; ++i) e[i] = ev / 3.0;
     }
     return;

@@ -19,6 +19,7 @@
 //   assembled analytically in one pass.
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "gm_common.h"
@@ -435,6 +436,8 @@ __global__ void __launch_bounds__(128) k_particle_stage(gm_grid g, gm_stage_args
   mp.psi = a.psi;
   mp.coh = a.p_coh ? a.p_coh[p] : a.coh;
   mp.shape = a.shape;
+  mp.ns = a.ns;
+  mp.lnJ = log(J);
   M3 tau, bn;
   SpectralState ss;
   double alpha_o = a.alpha_old ? a.alpha_old[p] : 0.0;
@@ -604,6 +607,19 @@ __global__ void __launch_bounds__(128) k_particle_stage(gm_grid g, gm_stage_args
 #pragma unroll
           for (int j = 0; j < 3; ++j) db.a[i][j] = (i == aa ? Bnh.a[bb][j] : 0.0) + (j == aa ? Bnh.a[bb][i] : 0.0);
         dtau = spectral_dtau(ss, db);
+        if (a.mat_kind == MAT_NORSAND) {
+          // the return also depends on ln J (specific volume): d ln J / d H_ab = A_ba
+          const double dlnJ = A.a[bb][aa];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              double s = 0.0;
+#pragma unroll
+              for (int k = 0; k < 3; ++k) s += ss.V.a[i][k] * ss.TJ[k] * ss.V.a[j][k];
+              dtau.a[i][j] += dlnJ * s;
+            }
+        }
       }
       // dP = -V0 [ dA^T M + A^T (dtau - p dJ I) ],  (dA^T X)[i][c] = -A[b][i] (A^T X)[a][c]
 #pragma unroll
@@ -930,6 +946,154 @@ __global__ void __launch_bounds__(64, 3) k_node_jacobian(gm_grid g, const double
   }
 }
 
+// 3D u-p node Jacobian: one 128-lane workgroup per block row I.  The candidate
+// particles of row I come in 27 base bins b = I - o; every particle of bin o
+// reaches the same 3x3x3 box of columns J = b + k, so lane (k, c) (108 of 128
+// lanes: 27 columns x 4 column dofs) owns entry column c of block (I, b + k) for
+// the whole bin and adds the bin's particles to it in bin order.  The row's 125
+// window blocks are accumulated in LDS (window-bit order), loaded into registers
+// per bin and stored back, so every entry receives the same sums in the same
+// order as in the one-wavefront-per-row form (k_node_jacobian) -- bit for bit the
+// same matrix -- while no lane waits on columns outside the particle's support.
+template <typename TD, typename TV>
+__global__ void __launch_bounds__(128) k_node_jacobian_up3(gm_grid g, const double* x_s, const double* lp_s,
+                                                           const int* cell_start, const uint8_t* ext_s,
+                                                           const TD* D, const uint8_t* freem, const int* row_ptr,
+                                                           const uint64_t* row_mask, TV* vals) {
+  typedef Lay<3, true> L;
+  constexpr int NB = 4, NZ = L::NZ, NY = L::NY, T = 128, JB = 8;
+  __shared__ double ACC[125 * NB * NB];
+  __shared__ double E[JB][NB * NZ];
+  __shared__ Axis AX[JB][3];
+  __shared__ int INS[JB];
+  __shared__ uint8_t EXS[JB];
+  const int I = blockIdx.x;
+  if (I >= g.n_nodes) return;
+  const int t = threadIdx.x;
+  for (int q = t; q < 125 * NB * NB; q += T) ACC[q] = 0.0;
+  int ci[3];
+  node_coords<3>(g, I, ci);
+  const int kk = t >> 2, c = t & 3;
+  const int kx = kk % 3, ky = (kk / 3) % 3, kz = kk / 9;
+  const bool lane_on = t < 27 * NB;
+  const int ncell[3] = {g.nx, g.ny, g.nz};
+  __syncthreads();
+  for (int oz = 0; oz < 3; ++oz)
+    for (int oy = 0; oy < 3; ++oy)
+      for (int ox = 0; ox < 3; ++ox) {
+        const int o[3] = {ox, oy, oz};
+        int bc[3];
+        bool ok = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          bc[a] = ci[a] - o[a];
+          if (bc[a] < 0 || bc[a] > ncell[a]) ok = false;
+        }
+        if (!ok) continue;  // uniform across the workgroup
+        const int bn = node_index<3>(g, bc);
+        const int ks = cell_start[bn], ke = cell_start[bn + 1];
+        // window bit of column J = b + k in row I (offset k - o per axis)
+        const int bit = (kx - ox + 2) + 5 * (ky - oy + 2) + 25 * (kz - oz + 2);
+        for (int k0 = ks; k0 < ke; k0 += JB) {
+          const int nj = min(JB, ke - k0);
+          if (t < nj) {
+            const int k = k0 + t;
+            const uint8_t ex = ext_s[k];
+            bool in = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) in = in && (o[a] < (((ex >> a) & 1) ? 3 : 2));
+            INS[t] = in ? 1 : 0;
+            EXS[t] = ex;
+            Axis ax[3];
+            load_axes<3>(g, x_s, lp_s, k, ax);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) AX[t][a] = ax[a];
+          }
+          __syncthreads();
+          // E = L_I D_p (NB x NZ) of each particle of the batch (same expressions as k_node_jacobian)
+          for (int q = t; q < nj * NB * NZ; q += T) {
+            const int pj = q / (NB * NZ), qq = q % (NB * NZ);
+            if (!INS[pj]) continue;
+            double wI, GI[3], waI;
+            node_geom<3>(AX[pj], o, wI, GI, waI);
+            const TD* Dp = D + (long)(k0 + pj) * NY * NZ;
+            const int r = qq / NZ, z = qq % NZ;
+            double sE = 0.0;
+            if (r < 3) {
+#pragma unroll
+              for (int bb = 0; bb < 3; ++bb) sE += GI[bb] * (double)Dp[(bb * 3 + r) * NZ + z];
+              sE += wI * (double)Dp[(L::Y_T + r) * NZ + z];
+            } else {
+              sE = wI * (double)Dp[L::Y_Q * NZ + z];
+#pragma unroll
+              for (int bb = 0; bb < 3; ++bb) sE += GI[bb] * (double)Dp[(L::Y_M + bb) * NZ + z];
+              sE += (wI - waI) * (double)Dp[L::Y_S * NZ + z];
+            }
+            E[pj][qq] = sE;
+          }
+          __syncthreads();
+          if (lane_on) {
+            double acc[NB];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) acc[r] = ACC[bit * NB * NB + r * NB + c];
+            const int kJ[3] = {kx, ky, kz};
+            for (int pj = 0; pj < nj; ++pj) {
+              if (!INS[pj]) continue;
+              bool inJ = true;
+#pragma unroll
+              for (int a = 0; a < 3; ++a) inJ = inJ && (kJ[a] < (((EXS[pj] >> a) & 1) ? 3 : 2));
+              if (!inJ) continue;
+              double wJ, GJ[3], waJ;
+              node_geom<3>(AX[pj], kJ, wJ, GJ, waJ);
+#pragma unroll
+              for (int r = 0; r < NB; ++r) {
+                double sv;
+                if (c < 3) {
+                  sv = 0.0;
+#pragma unroll
+                  for (int aa = 0; aa < 3; ++aa) sv += E[pj][r * NZ + aa * 3 + c] * GJ[aa];
+                  if (c == 2) sv += E[pj][r * NZ + L::Z_DU] * wJ;
+                } else {
+                  sv = E[pj][r * NZ + L::Z_P] * wJ + E[pj][r * NZ + L::Z_PB] * waJ;
+#pragma unroll
+                  for (int aa = 0; aa < 3; ++aa) sv += E[pj][r * NZ + L::Z_G + aa] * GJ[aa];
+                }
+                acc[r] += sv;
+              }
+            }
+#pragma unroll
+            for (int r = 0; r < NB; ++r) ACC[bit * NB * NB + r * NB + c] = acc[r];
+          }
+          __syncthreads();
+        }
+      }
+  // flush the window blocks present in the row pattern: masks, sign (-dR/du), identity rows
+  const uint64_t m0 = row_mask[2 * I], m1 = row_mask[2 * I + 1];
+  const int rp = row_ptr[I];
+  for (int b = t; b < 125; b += T) {
+    const uint64_t w = (b < 64) ? m0 : m1;
+    const int sh = b & 63;
+    if (!((w >> sh) & 1ull)) continue;
+    const int below = (b < 64) ? __popcll(sh ? (m0 & ((1ull << sh) - 1ull)) : 0ull)
+                               : __popcll(m0) + __popcll(sh ? (m1 & ((1ull << sh) - 1ull)) : 0ull);
+    const int cj[3] = {ci[0] + b % 5 - 2, ci[1] + (b / 5) % 5 - 2, ci[2] + b / 25 - 2};
+    const int Jn = node_index<3>(g, cj);
+    const bool diag = (Jn == I);
+    TV* out = vals + (long)(rp + below) * NB * NB;
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      const bool fr = freem[I * NB + r];
+#pragma unroll
+      for (int cc = 0; cc < NB; ++cc) {
+        const bool fc = freem[Jn * NB + cc];
+        double v = (fr && fc) ? -ACC[b * NB * NB + r * NB + cc] : 0.0;
+        if (diag && r == cc && !fr) v = 1.0;
+        out[r * NB + cc] = (TV)v;
+      }
+    }
+  }
+}
+
 extern "C" int gm_node_jacobian(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, const void* D,
                                 int tangent_fp32, const uint8_t* freem, void* vals, int vals_fp32, int coupled,
                                 hipStream_t s) {
@@ -947,13 +1111,105 @@ extern "C" int gm_node_jacobian(const gm_grid* g, const gm_bins* b, const gm_pat
   } while (0)
   if (g->dim == 2 && !coupled) GM_JAC_T(2, false);
   else if (g->dim == 2) GM_JAC_T(2, true);
-  else if (coupled) GM_JAC_T(3, true);
+  else if (coupled && !getenv("GM_JACOBIAN_ROW_WAVE")) {
+    // lanes per (column, column dof) of a bin (k_node_jacobian_up3); GM_JACOBIAN_ROW_WAVE=1
+    // selects the one-wavefront-per-row form, which assembles the same matrix
+#define GM_JAC3(TD, TV)                                                                                   \
+  k_node_jacobian_up3<TD, TV><<<n, 128, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, b->ext_s, (const TD*)D, \
+                                                freem, pat->row_ptr, pat->row_mask, (TV*)vals)
+    if (tangent_fp32 && vals_fp32) GM_JAC3(float, float);
+    else if (tangent_fp32) GM_JAC3(float, double);
+    else if (vals_fp32) GM_JAC3(double, float);
+    else GM_JAC3(double, double);
+#undef GM_JAC3
+  } else if (coupled) GM_JAC_T(3, true);
   else {
     gm::set_error("gm_node_jacobian: 3D single-phase is not a supported form");
     return -1;
   }
 #undef GM_JAC_T
 #undef GM_JAC
+  GM_LAUNCH_CHECK();
+  return 0;
+}
+
+// ================================================================== indenter penalty, preconditioner copy
+// The reference's indenter term r_z += -(h - (z + du_z)) pen_par w_I (mpm_utils.py:1240-1246)
+// enters the Newton matrix K = -dR/du as -pen_par w_I w_J on the z-dofs of every
+// penetrating particle's support: a negative stiffness that outweighs the soil's
+// under the indenter and makes K indefinite there.  The multigrid hierarchy is
+// built from a copy of K with that term's sign reversed (coef = 2 adds
+// 2 pen_par w_I w_J), which is positive definite under the indenter; the Krylov
+// operator keeps the exact K.  One thread per node row of the box
+// [lo, lo + cnt) that holds every support node of the penetrating particles;
+// the particles and the predicate are those of the particle stage
+// (x_s is the bin-sorted copy of x), and only free z-dofs are touched.
+template <typename TV>
+__global__ void __launch_bounds__(128) k_penalty_pc(gm_grid g, const double* x_s, const double* lp_s,
+                                                    const int* cell_start, const uint8_t* ext_s,
+                                                    const uint8_t* freem, const int* row_ptr,
+                                                    const uint64_t* row_mask, TV* vals, int lo0, int lo1, int lo2,
+                                                    int c0, int c1, int c2, double fx0, double fx1, double fy0,
+                                                    double fy1, double height, double coef) {
+  const int NB = 4;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c0 * c1 * c2) return;
+  int ci[3] = {lo0 + t % c0, lo1 + (t / c0) % c1, lo2 + t / (c0 * c1)};
+  if (ci[0] > g.nx || ci[1] > g.ny || ci[2] > g.nz) return;
+  int I = node_index<3>(g, ci);
+  if (!freem[I * NB + 2]) return;
+  const int rs = row_ptr[I];
+  const uint64_t m0 = row_mask[2 * I], m1 = row_mask[2 * I + 1];
+  for_candidates<3>(g, ci, cell_start, ext_s, [&](int k, const int* o, const int* b, bool in) {
+    if (!in) return;
+    double xx = x_s[3 * k], xy = x_s[3 * k + 1], xz = x_s[3 * k + 2];
+    if (!(xz >= height && xx > fx0 && xx < fx1 && xy > fy0 && xy < fy1)) return;
+    Axis ax[3];
+    load_axes<3>(g, x_s, lp_s, k, ax);
+    double wI = sel3(ax[0].w, o[0]) * sel3(ax[1].w, o[1]) * sel3(ax[2].w, o[2]);
+    for (int kz = 0; kz < ax[2].ext; ++kz)
+      for (int ky = 0; ky < ax[1].ext; ++ky)
+        for (int kx = 0; kx < ax[0].ext; ++kx) {
+          int cj[3] = {b[0] + kx, b[1] + ky, b[2] + kz};
+          int Jn = node_index<3>(g, cj);
+          if (!freem[Jn * NB + 2]) continue;
+          double wJ = sel3(ax[0].w, kx) * sel3(ax[1].w, ky) * sel3(ax[2].w, kz);
+          int bit = (kx - o[0] + 2) + 5 * (ky - o[1] + 2) + 25 * (kz - o[2] + 2);
+          // compressed slot: set window bits below `bit`
+          int below;
+          if (bit < 64) {
+            below = __popcll(bit ? (m0 & ((1ull << bit) - 1ull)) : 0ull);
+          } else {
+            int hb = bit - 64;
+            below = __popcll(m0) + __popcll(hb ? (m1 & ((1ull << hb) - 1ull)) : 0ull);
+          }
+          long slot = (long)(rs + below) * (NB * NB) + 2 * NB + 2;
+          vals[slot] = (TV)((double)vals[slot] + coef * wI * wJ);
+        }
+  });
+}
+
+extern "C" int gm_penalty_pc(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, const uint8_t* freem,
+                             void* vals, int vals_fp32, const int* box_lo, const int* box_cnt,
+                             const double* footing, double height, double coef, hipStream_t s) {
+  GM_REQUIRE(g && b && pat && freem && vals && box_lo && box_cnt && footing, "gm_penalty_pc: null argument");
+  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_penalty_pc: the indenter term exists in the 3D u-p form only");
+  long cnt = (long)box_cnt[0] * box_cnt[1] * box_cnt[2];
+  GM_REQUIRE(box_lo[0] >= 0 && box_lo[1] >= 0 && box_lo[2] >= 0 && box_cnt[0] >= 0 && box_cnt[1] >= 0 &&
+                 box_cnt[2] >= 0 && cnt < (1L << 30),
+             "gm_penalty_pc: bad node box");
+  if (cnt == 0) return 0;
+  int blk = gm_blocks(cnt, 128);
+  if (vals_fp32)
+    k_penalty_pc<float><<<blk, 128, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, b->ext_s, freem, pat->row_ptr,
+                                            pat->row_mask, (float*)vals, box_lo[0], box_lo[1], box_lo[2],
+                                            box_cnt[0], box_cnt[1], box_cnt[2], footing[0], footing[1],
+                                            footing[2], footing[3], height, coef);
+  else
+    k_penalty_pc<double><<<blk, 128, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, b->ext_s, freem, pat->row_ptr,
+                                             pat->row_mask, (double*)vals, box_lo[0], box_lo[1], box_lo[2],
+                                             box_cnt[0], box_cnt[1], box_cnt[2], footing[0], footing[1],
+                                             footing[2], footing[3], height, coef);
   GM_LAUNCH_CHECK();
   return 0;
 }
@@ -1256,111 +1512,24 @@ extern "C" int gm_return_map(int n, int mat_kind, int dp_apex, double lam, doubl
 }
 
 // ---------------------------------------------------------------------------------------------
-// NorSand return mapping (triaxial_NorSand.py:155-400), one particle per lane.  Pressure-dependent
-// hyperelasticity P = p0 exp(-(eps_v - ev0)/kappa), Q = 3 mu eps_s; yield Q + eta(P, pi) P; fixed
-// 15-iteration local Newton on (eps_v, eps_s, dlambda, pi_i) with hardening h (pi_i - pi*).
-// Tension (eps_v > tol) returns zero strain; no return when the yield value is below tol.
+// NorSand return mapping (triaxial_NorSand.py:155-400), one particle per lane (gm_device.h
+// return_norsand: pressure-dependent hyperelasticity P = p0 exp(-(eps_v - ev0)/kappa), Q = 3 mu
+// eps_s; yield Q + eta(P, pi) P; fixed 15-iteration local Newton on (eps_v, eps_s, dlambda, pi_i)
+// with hardening h (pi_i - pi*); tension (eps_v > tol) returns zero strain).
 // ---------------------------------------------------------------------------------------------
 namespace {
-
-__device__ inline double ns_eta(double M, double N, double P, double pi) {
-  if (fabs(N) < 1e-10) return M * (1.0 + log(pi / P));
-  return (M / N) * (1.0 - (1.0 - N) * pow(P / pi, N / (1.0 - N)));
-}
-__device__ inline double ns_deta_dP(double M, double N, double P, double pi) {
-  if (fabs(N) < 1e-10) return -M / P;
-  return -M * pow(P / pi, (2.0 * N - 1.0) / (1.0 - N)) / pi;
-}
-__device__ inline double ns_deta_dpi(double M, double N, double P, double pi) {
-  if (fabs(N) < 1e-10) return M / pi;
-  return M * pow(P / pi, (2.0 * N - 1.0) / (1.0 - N)) * P / (pi * pi);
-}
-__device__ inline double ns_eps_s(const double* e) {
-  double a = e[0] - e[1], b = e[1] - e[2], c = e[0] - e[2];
-  return sqrt(2.0 / 9.0 * (a * a + b * b + c * c));
-}
-
 __global__ void k_return_map_norsand(int n, gm_norsand_params q, const double* etr, const double* ev_total,
                                      const double* pi_old, double* eps, double* pi_new, double* res_last) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const double et[3] = {etr[3 * p], etr[3 * p + 1], etr[3 * p + 2]};
-  const double po = pi_old[p];
-  const double alpha_bar = -3.5 / q.beta;
-  const double ev = et[0] + et[1] + et[2];
-  const double es = ns_eps_s(et);
-  double nd[3] = {0.0, 0.0, 0.0};
-  if (es > q.tol) {
-    const double s23 = sqrt(2.0 / 3.0);
-    for (int i = 0; i < 3; ++i) nd[i] = s23 * (et[i] - ev / 3.0) / es;
-  }
-  double Pt = q.p0 * exp(-(ev - q.ev0) / q.kappa);
-  double Qt = 3.0 * q.mu * es;
-  const double new_v = exp(ev_total[p]) * q.v0;
-  double e[3] = {et[0], et[1], et[2]};
-  double pi = po;
-  double r[4] = {0.0, 0.0, 0.0, 0.0};
-  if (ev > q.tol) {
-    e[0] = e[1] = e[2] = 0.0;
-  } else if (Qt + ns_eta(q.M, q.N, Pt, po) * Pt >= q.tol) {
-    double dl = 0.0;
-    const double M = q.M, N = q.N;
-    for (int it = 0; it < 15; ++it) {
-      const double erv = e[0] + e[1] + e[2];
-      const double ers = ns_eps_s(e);
-      const double dEdP = ns_deta_dP(M, N, Pt, pi);
-      const double dEdPi = ns_deta_dpi(M, N, Pt, pi);
-      const double eta = ns_eta(M, N, Pt, pi);
-      const double dFdP = (eta - M) / (1.0 - N);
-      const double psi_i = new_v - q.vc0 + q.tilde_lambda * log(-pi);
-      // pi* / P: the yield-test tolerance picks the branch for pi*, 1e-10 for its derivative
-      // (triaxial_NorSand.py:356-362 and the derivative helpers)
-      const double pstar = fabs(N) < q.tol ? Pt * exp(alpha_bar * psi_i / M)
-                                           : Pt * pow(1.0 - alpha_bar * psi_i * N / M, (N - 1.0) / N);
-      const double dPs_dP = fabs(N) < 1e-10 ? exp(alpha_bar * psi_i / M)
-                                            : pow(1.0 - alpha_bar * psi_i * N / M, (N - 1.0) / N);
-      r[0] = erv - ev + dl * q.beta * dFdP;
-      r[1] = ers - es + dl;
-      r[2] = Qt + eta * Pt;
-      r[3] = pi - po + q.h * (pi - pstar) * dl;
-      const double dPdev = q.p0 * exp(-(erv - q.ev0) / q.kappa) / (-q.kappa);
-      const double d2F_dPdev = dEdP * dPdev / (1.0 - N);
-      const double d2F_dPdpi = dEdPi / (1.0 - N);
-      const double dFdev = (dEdP * Pt + eta) * dPdev;
-      const double dFdes = 3.0 * q.mu;
-      const double dFdpi = Pt * dEdPi;
-      const double dPsi_dpi = q.tilde_lambda / pi;
-      double dPs_dpi;
-      if (fabs(N) < 1e-10)
-        dPs_dpi = Pt * exp(alpha_bar * psi_i / M) * alpha_bar / M * dPsi_dpi;
-      else
-        dPs_dpi = Pt * (N - 1.0) / N * pow(1.0 / (1.0 - alpha_bar * psi_i * N / M), 1.0 / N) *
-                  (-alpha_bar * N / M) * dPsi_dpi;
-      const double dHdev = -q.h * dl * dPs_dP * dPdev;
-      const double dHdpi = 1.0 + q.h * dl * (1.0 - dPs_dpi);
-      double J[4][4] = {{1.0 + dl * q.beta * d2F_dPdev, 0.0, q.beta * dFdP, dl * q.beta * d2F_dPdpi},
-                        {0.0, 1.0, 1.0, 0.0},
-                        {dFdev, dFdes, 0.0, dFdpi},
-                        {dHdev, 0.0, q.h * (pi - pstar), dHdpi}};
-      double x[4] = {r[0], r[1], r[2], r[3]};
-      solve4(J, x);
-      const double nev = erv - x[0];
-      const double nes = ers - x[1];
-      dl -= x[2];
-      pi -= x[3];
-      const double s32 = sqrt(1.5);
-      for (int i = 0; i < 3; ++i) e[i] = nev / 3.0 + nes * nd[i] * s32;
-      const double erv2 = e[0] + e[1] + e[2];
-      Pt = q.p0 * exp(-(erv2 - q.ev0) / q.kappa);
-      Qt = 3.0 * q.mu * ns_eps_s(e);
-    }
-  }
+  double e[3], pn, r[4];
+  return_norsand(et, ev_total[p], pi_old[p], q, e, pn, r, nullptr, nullptr);
   for (int i = 0; i < 3; ++i) eps[3 * p + i] = e[i];
-  pi_new[p] = pi;
+  pi_new[p] = pn;
   if (res_last)
     for (int i = 0; i < 4; ++i) res_last[4 * p + i] = r[i];
 }
-
 }  // namespace
 
 extern "C" int gm_return_map_norsand(int n, const gm_norsand_params* prm, const double* eps_tr,

@@ -41,6 +41,15 @@ typedef struct gm_pattern {
   int nnzb;
 } gm_pattern;
 
+/* NorSand parameters (triaxial_NorSand.py:243-262): kappa, tilde_lambda, M, N,
+   beta, critical specific volume vc0, reference pressure p0 (negative in
+   compression), initial specific volume v0, hardening modulus h, shear modulus
+   mu, initial volumetric strain ev0, and the tolerance of the tension / yield
+   tests. */
+typedef struct gm_norsand_params {
+  double kappa, tilde_lambda, M, N, beta, vc0, p0, v0, h, mu, ev0, tol;
+} gm_norsand_params;
+
 /* Arguments of the per-particle stress/tangent stage. */
 typedef struct gm_stage_args {
   int n_particles;
@@ -75,6 +84,7 @@ typedef struct gm_stage_args {
   int penalty;           /* 3D indenter active */
   const double* p_phi;   /* per-particle Drucker-Prager friction angle, may be null (then phi) */
   const double* p_coh;   /* per-particle cohesion, may be null (then coh) */
+  gm_norsand_params ns;  /* NorSand (mat_kind 4): alpha_old/new carry the image pressure pi */
 } gm_stage_args;
 
 /* Krylov workspace (device pointers).  Vectors hold n_rows*nb entries of the
@@ -87,14 +97,6 @@ typedef struct gm_krylov_ws {
   double* sc;     /* >= 16 */
 } gm_krylov_ws;
 
-/* NorSand parameters (triaxial_NorSand.py:243-262): kappa, tilde_lambda, M, N,
-   beta, critical specific volume vc0, reference pressure p0 (negative in
-   compression), initial specific volume v0, hardening modulus h, shear modulus
-   mu, initial volumetric strain ev0, and the tolerance of the tension / yield
-   tests. */
-typedef struct gm_norsand_params {
-  double kappa, tilde_lambda, M, N, beta, vc0, p0, v0, h, mu, ev0, tol;
-} gm_norsand_params;
 
 /* Arguments of the 2D plane-strain local map used by the adjoint
    (gm_local_eval_2d / gm_local_vjp_2d).  Particle arrays are in particle order;

@@ -125,8 +125,12 @@ def return_mapping_DruckerPrager(eps_tr, lam_l, mu, phi, psi, c, shape=0.0, apex
     other than 10 exists for tests that separate converged returns from
     unconverged ones).  No return
     when f < 1e-10 or when P_trial >= 0 (the reference skips the apex,
-    :345-349).  ``apex='return'`` replaces that skip by a return to the apex
-    P = A/B (textbook; not in the reference).
+    :345-349).  ``apex='return'`` replaces that skip (not in the reference) by:
+    on the cone (shape 0), a return to the apex P = A/B when the closed-form
+    cone return would reverse the deviator; on the hyperbola (shape a > 0), a
+    cut-off to its tip (Q = 0, P = A (1 - a) / B) for trial states at or beyond
+    it, which a deviatoric flow (dilation 0) cannot return; other yielding
+    states take the local Newton.
     """
     n = eps_tr.shape[0]
     a_el = lam_l * np.ones((3, 3)) + 2.0 * mu * np.eye(3)
@@ -134,21 +138,20 @@ def return_mapping_DruckerPrager(eps_tr, lam_l, mu, phi, psi, c, shape=0.0, apex
     y = dp_yield(P, Q, phi, c, shape)
     eps = eps_tr.copy()
     plastic = (y >= 1e-10) & (P < 0.0)
-    if apex == 'return' and shape > 0.0:
-        # a hyperbolic surface has no apex: every yielding state returns to it
-        plastic = y >= 1e-10
-    elif apex == 'return':
+    if apex == 'return':
         A, B = _dp_AB(phi, c)
         _, Bg = _dp_AB(psi, c)
         K = lam_l + 2.0 * mu / 3.0
-        Sn = np.sqrt((S ** 2).sum(axis=1))
-        # cone return would reverse the deviator (closed form, shape factor 0) -> apex
-        dl_cone = y / (2.0 * mu + B * K * Bg)
-        tip = (y >= 1e-10) & ((P >= 0.0) | (2.0 * mu * dl_cone >= Sn))
+        if shape > 0.0:
+            tip = (y >= 1e-10) & (P >= A * (1.0 - shape) / B)
+        else:
+            Sn = np.sqrt((S ** 2).sum(axis=1))
+            dl_cone = y / (2.0 * mu + B * K * Bg)
+            tip = (y >= 1e-10) & (2.0 * mu * dl_cone >= Sn)
         if np.any(tip):
-            # volumetric strain giving P = A/B, zero deviator
-            eps[tip] = (A / B) / K / 3.0
-        plastic &= ~tip
+            # volumetric strain giving P = P_tip, zero deviator
+            eps[tip] = (A * (1.0 - shape) / B) / K / 3.0
+        plastic = (y >= 1e-10) & ~tip
     idx = np.nonzero(plastic)[0]
     if idx.size:
         e_tr = eps_tr[idx]

@@ -159,7 +159,7 @@ class MaterialParams:
     qs2d:83-90, qc3d:109-114, plus the north-star materials)."""
 
     def __init__(self, name, E, nu, kappa=0.0, H=0.0, phi=30.0, psi=0.0, cohesion=0.0, shape=0.0,
-                 dp_apex='reference'):
+                 dp_apex='reference', norsand=None):
         self.name = name
         self.E = E
         self.nu = nu
@@ -172,6 +172,26 @@ class MaterialParams:
         self.cohesion = cohesion
         self.shape = shape
         self.dp_apex = dp_apex
+        # NorSand (triaxial_NorSand.py:8-30 parameter names): kappa, tilde_lambda, M, N, beta, vc0, pc,
+        # p0, v0, h, G0, ev0 (elastic volumetric strain at which P = p0; 0: the undeformed state)
+        self.norsand = norsand_params(norsand) if name == 'NorSand' else None
+
+
+def norsand_params(d):
+    """NorSand parameters with the driver's defaults (triaxial_NorSand.py:8-30)."""
+    out = dict(kappa=0.0015, tilde_lambda=0.02, M=1.27, N=0.4, beta=1.0, vc0=1.8911, pc=-715.0, p0=-390.0,
+               v0=1.75, h=70.0, G0=4.5e4, ev0=0.0)
+    out.update(d or {})
+    out['mu'] = out['G0']
+    out['pi0'] = out['pc'] * (1.0 - out['N']) ** ((1.0 - out['N']) / out['N'])
+    return out
+
+
+def norsand_tau(e, prm):
+    """Principal Kirchhoff stress of NorSand's pressure-dependent hyperelasticity
+    (triaxial_NorSand.py:455-461): p0 exp(-(tr e - ev0)/kappa) + 2 G0 (e_i - tr e / 3)."""
+    tr = e.sum(axis=1, keepdims=True)
+    return prm['p0'] * np.exp(-(tr - prm['ev0']) / prm['kappa']) + 2.0 * prm['mu'] * (e - tr / 3.0)
 
 
 def constitutive(f3, F_old, b_old, mat, alpha_old):
@@ -199,6 +219,12 @@ def constitutive(f3, F_old, b_old, mat, alpha_old):
     elif mat.name == 'Drucker-Prager':
         er = om.return_mapping_DruckerPrager(eps, mat.lam, mat.mu, mat.phi, mat.psi, mat.cohesion,
                                              mat.shape, apex=mat.dp_apex)
+    elif mat.name == 'NorSand':
+        # alpha carries the image pressure pi; the specific volume follows the total volumetric
+        # strain ln J (the driver's trace of the total strain, triaxial_NorSand.py:279)
+        er, alpha_new, _ = om.return_mapping_NorSand(eps, np.log(J), alpha_old, mat.norsand)
+        tau = om.assemble_spectral(norsand_tau(er, mat.norsand), vec)
+        return F_new, J, tau / J[:, None, None], om.assemble_spectral(np.exp(2.0 * er), vec), alpha_new
     else:
         raise ValueError("Unimplemented material model: %s" % mat.name)
     tau_p = om.hencky_principal_tau(er, mat.lam, mat.mu)

@@ -156,7 +156,8 @@ class quasi_static_solver_2d(_NewtonMixin):
         self.mat = om.MaterialParams(m['material_name'], m['youngs_modulus'], m['poisson_ratio'],
                                      kappa=m.get('kappa', 0.0), H=m.get('H', 0.0), phi=m.get('friction_angle', 30.0),
                                      psi=m.get('dilation_angle', 0.0), cohesion=m.get('cohesion', 0.0),
-                                     shape=m.get('shape_factor', 0.0), dp_apex=m.get('dp_apex', 'reference'))
+                                     shape=m.get('shape_factor', 0.0), dp_apex=m.get('dp_apex', 'reference'),
+                                     norsand=m.get('norsand'))
         l = load_dict
         self.gravity_mag = l['gravity_mag']
         self.point_load = (l['point_load_value_x'], l['point_load_value_y'])
@@ -170,8 +171,10 @@ class quasi_static_solver_2d(_NewtonMixin):
         self.F_new = self.F_old.copy()
         self.b_old = self.F_old.copy()
         self.b_new = self.F_old.copy()
-        self.alpha_old = np.zeros(self.n_particles)
-        self.alpha_new = np.zeros(self.n_particles)
+        # hardening variable: J2's equivalent plastic strain, or NorSand's image pressure pi
+        a0 = self.mat.norsand['pi0'] if self.mat.norsand is not None else 0.0
+        self.alpha_old = np.full(self.n_particles, a0)
+        self.alpha_new = self.alpha_old.copy()
         self.sigma = np.zeros((self.n_particles, 3, 3))
         self.lp = og.init_lp(self.n_particles, self.GIMP_lp_initial, 2)
         self.external_flag = np.zeros(self.n_particles, dtype=bool)
@@ -259,7 +262,8 @@ class quasi_static_coupled_solver_2d(_NewtonMixin):
         self.mat = om.MaterialParams(m['material_name'], m['youngs_modulus'], m['poisson_ratio'],
                                      kappa=m.get('kappa', 0.0), H=m.get('H', 0.0), phi=m.get('friction_angle', 30.0),
                                      psi=m.get('dilation_angle', 0.0), cohesion=m.get('cohesion', 0.0),
-                                     shape=m.get('shape_factor', 0.0), dp_apex=m.get('dp_apex', 'reference'))
+                                     shape=m.get('shape_factor', 0.0), dp_apex=m.get('dp_apex', 'reference'),
+                                     norsand=m.get('norsand'))
         self.phi_initial = porous_media_dict['phi_initial']
         self.mobility_constant = porous_media_dict['mobility_constant']
         l = load_dict
@@ -276,8 +280,10 @@ class quasi_static_coupled_solver_2d(_NewtonMixin):
         self.F_new = self.F_old.copy()
         self.b_old = self.F_old.copy()
         self.b_new = self.F_old.copy()
-        self.alpha_old = np.zeros(self.n_particles)
-        self.alpha_new = np.zeros(self.n_particles)
+        # hardening variable: J2's equivalent plastic strain, or NorSand's image pressure pi
+        a0 = self.mat.norsand['pi0'] if self.mat.norsand is not None else 0.0
+        self.alpha_old = np.full(self.n_particles, a0)
+        self.alpha_new = self.alpha_old.copy()
         self.sigma = np.zeros((self.n_particles, 3, 3))
         self.pressure = np.zeros(self.n_particles)
         self.lp = og.init_lp(self.n_particles, self.GIMP_lp_initial, 2)
@@ -365,14 +371,17 @@ class quasi_static_coupled_solver_3d(_NewtonMixin):
         self.mat = om.MaterialParams(m['material_name'], m['youngs_modulus'], m['poisson_ratio'],
                                      kappa=m.get('kappa', 0.0), H=m.get('H', 0.0), phi=m.get('friction_angle', 30.0),
                                      psi=m.get('dilation_angle', 0.0), cohesion=m.get('cohesion', 0.0),
-                                     shape=m.get('shape_factor', 0.0), dp_apex=m.get('dp_apex', 'reference'))
+                                     shape=m.get('shape_factor', 0.0), dp_apex=m.get('dp_apex', 'reference'),
+                                     norsand=m.get('norsand'))
         self.phi_initial = porous_media_dict['phi_initial']
         self.mobility_constant = porous_media_dict['mobility_constant']
         l = load_dict
         self.gravity_mag = l['gravity_mag']
         self.footing = (l['footing_region_min_x'], l['footing_region_max_x'], l['footing_region_min_y'],
                         l['footing_region_max_y'], l['footing_initial_height'], l['footing_incr_disp'],
-                        l['penalty_factor'])
+                        # the reference's sign (mpm_utils.py:1240-1246), or a negative factor for
+                        # the textbook penalty that pushes a penetrating particle down
+                        l['penalty_factor'] * (1.0 if l.get('penalty_form', 'reference') == 'reference' else -1.0))
         self.set_boundary_dofs = set_boundary_dofs
         if x_particles is None:
             x_particles = om.init_particles_rectangle_3d(*self.start, *self.end, self.dx, *self.n_grid, self.PPD)
@@ -382,8 +391,10 @@ class quasi_static_coupled_solver_3d(_NewtonMixin):
         self.F_new = self.F_old.copy()
         self.b_old = self.F_old.copy()
         self.b_new = self.F_old.copy()
-        self.alpha_old = np.zeros(self.n_particles)
-        self.alpha_new = np.zeros(self.n_particles)
+        # hardening variable: J2's equivalent plastic strain, or NorSand's image pressure pi
+        a0 = self.mat.norsand['pi0'] if self.mat.norsand is not None else 0.0
+        self.alpha_old = np.full(self.n_particles, a0)
+        self.alpha_new = self.alpha_old.copy()
         self.sigma = np.zeros((self.n_particles, 3, 3))
         self.pressure = np.zeros(self.n_particles)
         self.lp = og.init_lp(self.n_particles, self.GIMP_lp_initial, 3)

@@ -354,3 +354,35 @@ def test_e_grad_matches_fd_of_loss_at_fixed_solution():
     lm_ = get_indentation_loss(force(E - h), l['footing_incr_disp'], 2.0)
     fd = (lp_ - lm_) / (2 * h)
     assert abs(float(s.E_grad[0]) - fd) <= 1e-6 * abs(fd)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('shape', [0.0, 0.1])
+def test_return_mapping_dp_dilation_zero_tension_matches_oracle(shape):
+    """Device Drucker-Prager return (phi 30, c 5, psi 0, apex return) on trial states
+    around the tension region -- returns to the surface below the tip, cut-off at
+    or beyond it -- against the oracle, and its consistent tangent against central
+    differences of the oracle away from the regime boundaries."""
+    from geowarp_amd.materials import plasticity as pl
+    import test_oracle as T
+    e, lam, mu, K, A, B, p_tip = T._dp_psi0_cases(shape)
+    ref = om.return_mapping_DruckerPrager(e, lam, mu, 30.0, 0.0, 5.0, shape, apex='return', n_iter=30)
+    out, dE = pl.return_mapping_DruckerPrager(torch.from_numpy(e).to(_dev()), lam, mu, 30.0, 0.0, 5.0, shape,
+                                              apex='return', tangent=True)
+    out = out.cpu().numpy()
+    np.testing.assert_allclose(out, ref, rtol=1e-10, atol=1e-13)
+    h = 1e-9
+    fd = np.zeros((e.shape[0], 3, 3))
+    for j in range(3):
+        ep, em = e.copy(), e.copy()
+        ep[:, j] += h
+        em[:, j] -= h
+        fd[:, :, j] = (om.return_mapping_DruckerPrager(ep, lam, mu, 30.0, 0.0, 5.0, shape, apex='return', n_iter=30)
+                       - om.return_mapping_DruckerPrager(em, lam, mu, 30.0, 0.0, 5.0, shape, apex='return',
+                                                         n_iter=30)) / (2 * h)
+    _, P, S, Q = om._dp_PQ(e, lam, mu)
+    y = om.dp_yield(P, Q, 30.0, 5.0, shape)
+    # away from the elastic/plastic boundary and from the tip cut-off boundary
+    away = (np.abs(y) > 1e-3) & (np.abs(P - p_tip) > 1e-2)
+    assert away.sum() > 200
+    np.testing.assert_allclose(dE.cpu().numpy().reshape(-1, 3, 3)[away], fd[away], atol=2e-5)

@@ -219,3 +219,44 @@ def test_golden_fixtures_load():
     for name in ['bar_nh', 'bar_hencky', 'bar_j2', 'bar_dp', 'terzaghi', 'indent_nh']:
         g = np.load(os.path.join(gdir, name + '.npz'))
         assert 'jac_val' in g and 's0_x' in g
+
+
+# ------------------------------------------------------------------ Drucker-Prager apex / tip (dilation 0)
+def _dp_psi0_cases(shape):
+    """Principal trial strains around the tension region of phi 30, c 5, psi 0."""
+    lam, mu = 11538.46, 7692.31
+    K = lam + 2 * mu / 3
+    A, B = om._dp_AB(30.0, 5.0)
+    p_tip = A * (1 - shape) / B
+    rng = np.random.default_rng(9)
+    P = rng.uniform(-3 * p_tip, 2 * p_tip, 400)
+    d = rng.normal(size=(400, 3))
+    d -= d.mean(1, keepdims=True)
+    d *= (rng.uniform(1e-4, 3e-3, 400) / np.linalg.norm(d, axis=1))[:, None]
+    e = P[:, None] / (3 * K) + d  # volumetric part gives P (Hencky: P = K tr e)
+    return e, lam, mu, K, A, B, p_tip
+
+
+@pytest.mark.parametrize('shape', [0.0, 0.1])
+def test_dp_dilation_zero_return_keeps_pressure_or_cuts_off_at_the_tip(shape):
+    """With dilation 0 the flow is deviatoric: a yielding trial state below the
+    tip keeps its pressure P and its deviator direction, and lands on the surface
+    at Q = sqrt(1.5 ((A - B P)^2 - a^2 A^2)) (closed form of the yield function);
+    a state at or beyond the tip (P >= A (1 - a)/B) is cut off to the tip, Q = 0.
+    Tensile states below the tip are returned to the surface, not to the apex."""
+    e, lam, mu, K, A, B, p_tip = _dp_psi0_cases(shape)
+    _, P, S, Q = om._dp_PQ(e, lam, mu)
+    y = om.dp_yield(P, Q, 30.0, 5.0, shape)
+    er = om.return_mapping_DruckerPrager(e, lam, mu, 30.0, 0.0, 5.0, shape, apex='return')
+    _, P2, S2, Q2 = om._dp_PQ(er, lam, mu)
+    plastic = y >= 1e-10
+    tip = plastic & (P >= p_tip) if shape > 0 else plastic & (y >= np.linalg.norm(S, axis=1))
+    cone = plastic & ~tip
+    assert tip.sum() > 20 and cone.sum() > 50 and (cone & (P > 0)).sum() > 10
+    assert np.allclose(P2[tip], p_tip, rtol=1e-12) and np.all(np.abs(Q2[tip]) < 1e-9)
+    assert np.allclose(P2[cone], P[cone], rtol=1e-9, atol=1e-9)
+    q_ref = np.sqrt(1.5 * ((A - B * P[cone]) ** 2 - (shape * A) ** 2))
+    assert np.allclose(Q2[cone], q_ref, rtol=1e-8)
+    cosang = (S2[cone] * S[cone]).sum(1) / (np.linalg.norm(S2[cone], axis=1) * np.linalg.norm(S[cone], axis=1))
+    assert np.allclose(cosang, 1.0, atol=1e-10)
+    assert np.array_equal(er[~plastic], e[~plastic])
