@@ -1613,10 +1613,10 @@ struct MGPrec {
 };
 }  // namespace mg
 
-extern "C" int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre,
-                           int nu_post, int coarse_sweeps, double omega, const void* b, void* x, void* V, void* Z,
-                           void* z, void* w, double* part, double* dh, int m, double tol, int maxiter, int* iters,
-                           double* relres, hipStream_t s) {
+extern "C" int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, const void* op_vals,
+                           int nu_pre, int nu_post, int coarse_sweeps, double omega, const void* b, void* x, void* V,
+                           void* Z, void* z, void* w, double* part, double* dh, int m, double tol, int maxiter,
+                           int* iters, double* relres, hipStream_t s) {
   int rc = mg_check(dim, nb, n_levels, levels, fp32);
   if (rc) return rc;
   GM_REQUIRE(iters && relres && m >= 1 && m <= 256 && b && x && V && Z && z && w, "gm_gmres_mg: bad argument");
@@ -1624,16 +1624,19 @@ extern "C" int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_
   mg::Cycle cy{dim, n_levels, nu_pre, nu_post, coarse_sweeps, omega, levels};
   const gm_mg_level& l0 = levels[0];
   long n = (long)l0.n_nodes * nb;
+  // the Krylov operator: op_vals (same pattern as level 0) when the hierarchy is built from
+  // a modified copy of the system matrix, else level 0's own matrix
+  const void* opv = op_vals ? op_vals : l0.vals;
 #define GM_GMG(D, B)                                                                                            \
   do {                                                                                                          \
     if (fp32) {                                                                                                 \
       mg::MGPrec<D, B, float> pc{cy, n};                                                                        \
-      return gmres_impl<B, float, float>(l0.n_nodes, l0.row_ptr, l0.col_idx, (const float*)l0.vals,             \
+      return gmres_impl<B, float, float>(l0.n_nodes, l0.row_ptr, l0.col_idx, (const float*)opv,                 \
                                          (const float*)b, (float*)x, (float*)V, (float*)z, (float*)w, pc, part, \
                                          dh, m, tol, maxiter, iters, relres, s, (float*)Z);                     \
     }                                                                                                           \
     mg::MGPrec<D, B, double> pc{cy, n};                                                                         \
-    return gmres_impl<B, double, double>(l0.n_nodes, l0.row_ptr, l0.col_idx, (const double*)l0.vals,            \
+    return gmres_impl<B, double, double>(l0.n_nodes, l0.row_ptr, l0.col_idx, (const double*)opv,                \
                                          (const double*)b, (double*)x, (double*)V, (double*)z, (double*)w, pc,  \
                                          part, dh, m, tol, maxiter, iters, relres, s, (double*)Z);              \
   } while (0)

@@ -120,6 +120,31 @@ def bar_j2(n_cells_x=256, n_cells_y=1024, dx=1.953125e-3, total_steps=50, tol=1e
     )
 
 
+def norsand_box(n_cells_x=4, n_cells_y=8, dx=0.0125, total_steps=10, strain=0.02, tol=1e-10, precision='fp64',
+                norsand=None):
+    """2D plane-strain NorSand specimen in a rigid box (x fixed on the side walls,
+    fixed base) compressed by a top platen by `strain` of its height
+    (oedometric loading), no gravity.  The particles start at the isotropic
+    NorSand pressure p0 at zero strain, in equilibrium with the walls; the
+    material is the one of the reference's NorSand driver
+    (triaxial_NorSand.py:8-30) unless `norsand` overrides keys."""
+    g = _grid2d(n_cells_x, n_cells_y, dx)
+    PPD = 2
+    npart = n_cells_x * n_cells_y * PPD * PPD
+    return dict(
+        solver='quasi_static_solver_2d',
+        background_grid_dict=g,
+        particles_dict=dict(n_particles=npart, PPD=PPD, p_vol=(dx / PPD) ** 2, p_rho=2.0),
+        iteration_dict=dict(n_iter=12, tol=tol, tol_mode='relative', precision=precision),
+        material_dict=dict(youngs_modulus=1.0e5, poisson_ratio=0.3, material_name='NorSand',
+                           norsand=dict(norsand or {})),
+        load_dict=dict(gravity_mag=0.0, traction_value_x=0.0, traction_value_y=0.0, point_load_value_x=0.0,
+                       point_load_value_y=0.0, prescribed_increment=make_platen(-strain * n_cells_y * dx)),
+        set_boundary_dofs=make_bar_bc(1, 1 + n_cells_x, 1),
+        total_steps=total_steps,
+    )
+
+
 def make_cantilever_bc(end_y, d0, dx):
     """cantilever_beam_cpGIMP.py:98-113: x fixed on node columns ix <= 1; y fixed
     on those columns at the node row int((end_y - d0/2 + dx/2)/dx) only."""
@@ -287,7 +312,8 @@ def make_indent_bc():
 
 def indent3d(n_cells=(64, 64, 32), dx=1.0 / 64, total_steps=40, depth=0.05, footing=0.2, material='Drucker-Prager',
              E=2.0e4, precision='fp64', tol=1e-8, n_iter=10, mobility=1e-6, dt=1.0, gravity=0.0,
-             dp_apex='reference', shape_factor=0.0, clip_top=True, krylov=None):
+             dp_apex='reference', shape_factor=0.0, clip_top=True, krylov=None, footing_start='reference',
+             penalty_form='reference'):
     ncx, ncy, ncz = n_cells
     nx, ny, nz = ncx + 4, ncy + 4, ncz + 3
     sx, sy, sz = dx, dx, 0.0
@@ -298,6 +324,12 @@ def indent3d(n_cells=(64, 64, 32), dx=1.0 / 64, total_steps=40, depth=0.05, foot
              domain_max_z=ez if clip_top else nz * dx)
     PPD = 2
     npart = ncx * ncy * ncz * PPD ** 3
+
+    # indenter face before load step 0: the driver's end_z - dz/PPD (3d_indentation_coupled_cpGIMP.py:84),
+    # which lies a quarter cell below the top particle row at 2 particles per cell, so that load
+    # step 0 pushes that quarter cell plus one increment; 'surface' starts at the top particle row
+    # so that every load step pushes exactly one increment (depth in total_steps steps)
+    h0 = {'reference': ez - dx / PPD, 'surface': ez - dx / (2 * PPD)}[footing_start]
 
     def flags(x, traction_flag, pbc_flag):
         top = x[:, 2] > ez - dx / PPD  # 3d_indentation_coupled_cpGIMP.py:348-358
@@ -315,8 +347,8 @@ def indent3d(n_cells=(64, 64, 32), dx=1.0 / 64, total_steps=40, depth=0.05, foot
         load_dict=dict(gravity_mag=gravity, traction_value_x=0.0, traction_value_y=0.0, traction_value_z=0.0,
                        point_load_value_x=0.0, point_load_value_y=0.0, point_load_value_z=0.0,
                        footing_region_min_x=sx, footing_region_max_x=sx + footing, footing_region_min_y=sy,
-                       footing_region_max_y=sy + footing, footing_initial_height=ez - dx / PPD,
-                       footing_incr_disp=-depth / total_steps, penalty_factor=1000.0),
+                       footing_region_max_y=sy + footing, footing_initial_height=h0,
+                       footing_incr_disp=-depth / total_steps, penalty_factor=1000.0, penalty_form=penalty_form),
         set_boundary_dofs=make_indent_bc(),
         particle_flags=flags,
         total_steps=total_steps,

@@ -13,6 +13,8 @@ Extensions over the reference (all optional, defaults reproduce it):
                                   FP32 Krylov, FP64 residual/refinement)
   iteration_dict['linear_tol'], ['linear_maxiter'], ['ir_steps'], ['forcing']
                                   ('fixed' | 'ew': Eisenstat-Walker inner tolerance),
+  ['final_forcing']               'ew' inner tolerance of the last correction (1e-3), the one
+                                  applied after the residual has passed the Newton test
   ['forcing_max']                 cap of the 'ew' inner tolerance (1e-2; 1e-4 for J2 and
                                   Drucker-Prager; a hundred times tighter after each Newton
                                   iteration that reduces the residual by less than half),
@@ -27,6 +29,8 @@ Extensions over the reference (all optional, defaults reproduce it):
   ['mixed_assembly']              mixed form: 'fp32' tangent and Jacobian (default) or 'fp64'
                                   (FP64 assembly, FP32 copy for the Krylov solve, refinement
                                   residual against the FP64 Jacobian)
+  ['abs_floor']                   the Newton test also passes once the residual norm is below it
+  ['substeps'] (3D u-p)           levels of load-step halving after a failed Newton iteration
   iteration_dict['verbose']       print residual norms like the reference
   material_dict['H']              J2 linear isotropic hardening modulus
   material_dict['friction_angle'], ['dilation_angle'], ['cohesion'],
@@ -39,6 +43,19 @@ import torch
 
 from .. import _lib as L
 from ._engine import Engine, MAT_KIND
+
+
+def norsand_params(overrides=None):
+    """NorSand material parameters: the reference driver's values
+    (triaxial_NorSand.py:8-30: kappa, tilde_lambda, M, N, beta, vc0, pc, p0, v0, h,
+    G0) updated by `overrides`, plus ev0 (the elastic volumetric strain at which
+    P = p0; 0 puts the undeformed particle at p0) and the initial image pressure
+    pi0 = pc (1 - N)^((1 - N)/N)."""
+    out = dict(kappa=0.0015, tilde_lambda=0.02, M=1.27, N=0.4, beta=1.0, vc0=1.8911, pc=-715.0, p0=-390.0, v0=1.75,
+               h=70.0, G0=4.5e4, ev0=0.0)
+    out.update(overrides or {})
+    out['pi0'] = out['pc'] * (1.0 - out['N']) ** ((1.0 - out['N']) / out['N'])
+    return out
 
 
 class DofStruct:
@@ -134,12 +151,17 @@ class _QuasiStaticBase:
         # the iteration then stalls on the changed active set (a 256 x 1024-cell J2 bar stalls
         # at relative residual 3e-2 with a 1e-2 cap in FP64 as in FP32, and converges
         # quadratically with 1e-4)
-        self.eta_max = it.get('forcing_max', 1e-4 if self.material_name in ('J2', 'Drucker-Prager') else 1e-2)
+        self.eta_max = it.get('forcing_max', 1e-4 if self.material_name in ('J2', 'Drucker-Prager', 'NorSand') else 1e-2)
         # mixed form: solve a Newton correction again in FP64 when the FP32 solve fails
         self.mixed_fallback = it.get('mixed_fallback', True)
         # Eisenstat-Walker choice 2 parameters: eta_k = gamma (|r_k| / |r_k-1|)^alpha
         self.ew_gamma = it.get('forcing_gamma', 0.9)
         self.ew_alpha = it.get('forcing_alpha', 2.0)
+        # inner tolerance of the last correction (the one applied after the residual has
+        # passed the Newton test) under Eisenstat-Walker forcing
+        self.final_forcing = it.get('final_forcing', 1e-3)
+        # absolute floor of the Newton test (None: the tolerance alone)
+        self.abs_floor = it.get('abs_floor')
         if self.material_name not in MAT_KIND:
             raise ValueError("Unimplemented material model: %s" % self.material_name)
         self.youngs_modulus = m['youngs_modulus']
@@ -152,6 +174,7 @@ class _QuasiStaticBase:
         self.cohesion = m.get('cohesion', 0.0)
         self.shape_factor = m.get('shape_factor', 0.0)
         self.dp_apex = {'reference': 0, 'return': 1}[m.get('dp_apex', 'reference')]
+        self.norsand = norsand_params(m.get('norsand')) if self.material_name == 'NorSand' else None
         self.gravity_mag = load_dict['gravity_mag']
         self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
         spacing = (self.dx, self.dx, 0.0) if self.dim == 2 else (self.dx, self.dy, self.dz)
@@ -198,8 +221,10 @@ class _QuasiStaticBase:
         for t in (self.deformation_gradient_total_new, self.deformation_gradient_total_old,
                   self.left_Cauchy_Green_new, self.left_Cauchy_Green_old):
             t.copy_(I.expand_as(t))
-        self.plastic_strain_new.zero_()
-        self.plastic_strain_old.zero_()
+        # hardening variable: J2's equivalent plastic strain (0), or NorSand's image pressure pi_0
+        a0 = self.norsand['pi0'] if self.norsand is not None else 0.0
+        self.plastic_strain_new.fill_(a0)
+        self.plastic_strain_old.fill_(a0)
         self.GIMP_lp.fill_(self.GIMP_lp_initial)
 
     # ------------------------------------------------------------------ stage arguments
@@ -232,6 +257,10 @@ class _QuasiStaticBase:
         ca = getattr(self, 'cohesion_array', None)
         a.p_phi = pa.data_ptr() if pa is not None else None
         a.p_coh = ca.data_ptr() if ca is not None else None
+        if self.norsand is not None:
+            q = self.norsand
+            a.ns = L.gm_norsand_params(q['kappa'], q['tilde_lambda'], q['M'], q['N'], q['beta'], q['vc0'], q['p0'],
+                                       q['v0'], q['h'], q['G0'], q['ev0'], 1e-10)
         a.p_vol, a.p_rho = self.p_vol, self.p_rho
         a.PPD = float(self.PPD)
         s1 = (float(current_step) + 1.0) / float(total_steps)
@@ -280,6 +309,11 @@ class _QuasiStaticBase:
                     self._eta_cap = max(self.linear_tol, 1e-2 * self._eta_cap)
                 eta = self._ew_forcing(nr, nr_prev, eta, r0 if r0 is not None else nr)
                 tol_lin = eta
+                if r0 is not None and self._passes(nr, r0):
+                    # this residual passes the test, so its correction is the last one applied
+                    # (reference ordering): solve it to final_forcing, so that the accepted state
+                    # lies well inside the tolerance rather than at it
+                    tol_lin = min(tol_lin, self.final_forcing)
             eng.solve(tol_lin, self.linear_maxiter, ir_steps=self.ir_steps, method=self.krylov_method,
                       restart=self.krylov_restart, equilibrate=self.equilibrate)
             if (eng.precision == 'mixed' and self.mixed_fallback and not self.equilibrate
@@ -296,7 +330,7 @@ class _QuasiStaticBase:
                 eng.solve_fp64(tol_lin, self.linear_maxiter, method=self.krylov_method, restart=self.krylov_restart)
             if tm is not None:
                 tm.mark('krylov')
-            converged_here = (nr if self.tol_mode == 'absolute' else (nr / r0 if r0 else 1.0)) < self.tol
+            converged_here = self._passes(nr, r0)
             if self.line_search and not converged_here and it + 1 < self.n_iter:
                 self._line_search(args, u_new, nr, after_residual)
             else:
@@ -312,11 +346,25 @@ class _QuasiStaticBase:
                 raise L.GMError('Newton residual is not finite at iteration %d' % it)
             if r0 is None:
                 r0 = nr
-            crit = nr if self.tol_mode == 'absolute' else (nr / r0 if r0 > 0 else 0.0)
-            if crit < self.tol:
+            passed = self._passes(nr, r0)
+            if passed:
                 break
         self.newton_iterations = it + 1
-        self.newton_converged = bool(self.residual_history) and crit < self.tol
+        self.newton_converged = bool(self.residual_history) and passed
+
+    def _passes(self, nr, r0):
+        """Newton test on the residual norm nr (r0: the first of the load step, None
+        before it is known): absolute (reference, qs2d:230) or relative to r0, and
+        with iteration_dict['abs_floor'] also passed once nr is below that floor (a
+        relative test cannot pass when r0 itself is near the residual's round-off
+        level)."""
+        if self.abs_floor is not None and nr < self.abs_floor:
+            return True
+        if self.tol_mode == 'absolute':
+            return nr < self.tol
+        if r0 is None:
+            return False
+        return (nr / r0 if r0 > 0 else 0.0) < self.tol
 
     def _line_search(self, args, u_new, nr, after_residual, max_halvings=6):
         """Backtracking on the Newton correction (not in the reference, which
@@ -359,6 +407,24 @@ class _QuasiStaticBase:
         if nr > 0:
             eta = max(eta, 0.5 * target / nr)
         return min(max(eta, self.linear_tol), getattr(self, '_eta_cap', self.eta_max))
+
+    # ------------------------------------------------------------------ checkpoint / restart
+    _STATE = ('x_particles', 'GIMP_lp', 'deformation_gradient_total_old', 'deformation_gradient_total_new',
+              'left_Cauchy_Green_old', 'left_Cauchy_Green_new', 'particle_Cauchy_stress_array',
+              'plastic_strain_old', 'plastic_strain_new')
+    _STATE_EXTRA = ()
+
+    def state_dict(self):
+        """Device copies of the particle state between load steps (positions, lp,
+        F and b_e old/new, stress, hardening variable and the form's extra
+        arrays, e.g. pore pressures): enough to restart the load path from here
+        with load_state_dict (the nodal state is rebuilt by every load step's
+        P2G)."""
+        return {k: getattr(self, k).detach().clone() for k in self._STATE + self._STATE_EXTRA}
+
+    def load_state_dict(self, state):
+        for k in self._STATE + self._STATE_EXTRA:
+            getattr(self, k).copy_(state[k])
 
     # ------------------------------------------------------------------ dof-major views
     def _sync_flags_out(self):

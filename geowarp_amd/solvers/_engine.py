@@ -21,7 +21,7 @@ import torch
 from .. import _lib as L
 from ._multigrid import Multigrid
 
-MAT_KIND = {'Hencky elasticity': 0, 'J2': 1, 'Neo-Hookean': 2, 'Drucker-Prager': 3}
+MAT_KIND = {'Hencky elasticity': 0, 'J2': 1, 'Neo-Hookean': 2, 'Drucker-Prager': 3, 'NorSand': 4}
 
 
 def _stream():
@@ -96,6 +96,12 @@ class Engine:
         # mixed form: 'fp32' assembles the tangent and the Jacobian in FP32; 'fp64' assembles
         # them in FP64, solves with an FP32 copy and refines against the FP64 Jacobian
         self.mixed_assembly = 'fp32'
+        # 3D indenter: dict(footing=(min_x, max_x, min_y, max_y), height, pen_par, lp_max) of the
+        # current load step, set by the solver; with penalty_pc the multigrid hierarchy is built
+        # from a copy of the Newton matrix whose penalty term has the opposite sign
+        self.penalty = None
+        self.penalty_pc = True
+        self.vals_pc = None
 
     # ------------------------------------------------------------------ buffers
     def _alloc_vals(self):
@@ -336,6 +342,34 @@ class Engine:
             its += its2
         return its, rel
 
+    def _hierarchy_matrix(self):
+        """Level-0 matrix of the multigrid hierarchy: the Newton matrix itself, or
+        (3D indenter, penalty_pc) a copy whose penalty term -pen_par w_I w_J on the
+        contact z-dofs is replaced by +pen_par w_I w_J (gm_penalty_pc).  Returns
+        (tensor, operator) with operator the Krylov matrix when it differs."""
+        pen = self.penalty
+        if pen is None or not self.penalty_pc or not (self.dim == 3 and self.coupled) or pen['pen_par'] <= 0.0:
+            return self.vals, None  # (a textbook penalty, pen_par < 0, already adds +k w_I w_J)
+        n = self.nnzb * self.nb * self.nb
+        if self.vals_pc is None or self.vals_pc.dtype != self.vals.dtype or self.vals_pc.numel() < n:
+            self.vals_pc = torch.zeros(self.vals.numel(), dtype=self.vals.dtype, device=self.dev)
+        self.vals_pc[:n].copy_(self.vals[:n])
+        g = self.grid
+        h = (g.h[0], g.h[1], g.h[2])
+        fx0, fx1, fy0, fy1 = pen['footing']
+        m = 3 + int(math.ceil(pen.get('lp_max', 0.5 * h[0]) / h[0]))
+        lo = [max(0, int(math.floor(fx0 / h[0])) - m), max(0, int(math.floor(fy0 / h[1])) - m),
+              max(0, int(math.floor(pen['height'] / h[2])) - m)]
+        hi = [min(g.nx, int(math.ceil(fx1 / h[0])) + m), min(g.ny, int(math.ceil(fy1 / h[1])) + m), g.nz]
+        cnt = [max(0, hi[a] - lo[a] + 1) for a in range(3)]
+        L.check(L.lib().gm_penalty_pc(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins),
+                                      L.ctypes.byref(self.pattern), L.ptr(self.freem), L.ptr(self.vals_pc),
+                                      int(self.vals.dtype == torch.float32), (L.c_int * 3)(*lo),
+                                      (L.c_int * 3)(*cnt), (L.c_double * 4)(fx0, fx1, fy0, fy1),
+                                      float(pen['height']), 2.0 * float(pen['pen_par']), _stream()),
+                'gm_penalty_pc')
+        return self.vals_pc, self.vals
+
     def _krylov(self, b, x, fp32, tol, maxiter, method='gmres', restart=40, check_every=8):
         """One Krylov solve K x = b on the current Jacobian: GMRES(restart)
         preconditioned by geometric-multigrid V-cycles ('mg') or block Jacobi
@@ -347,7 +381,8 @@ class Engine:
             entry = self.multigrid(fp32)
             mgh = entry[0]
             if entry[1] != self.jac_version:
-                mgh.setup(_stream())
+                hv, op = self._hierarchy_matrix() if not mgh.ublock else (self.vals, None)
+                mgh.setup(_stream(), vals=hv, op_vals=op)
                 entry[1] = self.jac_version
             V, z, w, dinv, part, dh = self.gmres_workspace(self.nb, fp32, restart)
             return mgh.gmres(b, x, V, z, w, part, dh, restart, tol, maxiter, _stream())

@@ -127,10 +127,14 @@ class Multigrid:
     def n_levels(self):
         return len(self.dims)
 
-    def setup(self, stream):
-        """Point level 0 at the current Jacobian and build the coarse operators."""
+    def setup(self, stream, vals=None, op_vals=None):
+        """Point level 0 at the current Jacobian (or at `vals`, a matrix in its
+        pattern the hierarchy is built from, with op_vals then the Krylov
+        operator) and build the coarse operators."""
         eng = self.eng
         lv = self.levels[0]
+        self.hier_vals = eng.vals if vals is None else vals
+        self.op_vals = op_vals
         lv.nnzb = eng.nnzb
         lv.row_ptr, lv.col_idx, lv.row_mask = eng.row_ptr.data_ptr(), eng.col_idx.data_ptr(), \
             eng.row_mask.data_ptr()
@@ -153,7 +157,7 @@ class Multigrid:
                                         self.n_levels, self.levels, L.ptr(self.ap), L.ptr(self.sinv), stream),
                     'gm_up_setup')
         else:
-            lv.vals, lv.freem = eng.vals.data_ptr(), eng.freem.data_ptr()
+            lv.vals, lv.freem = self.hier_vals.data_ptr(), eng.freem.data_ptr()
             L.check(L.lib().gm_mg_setup(self.dim, self.nb, int(self.fp32), self.n_levels, self.levels,
                                         L.ptr(self.ap), stream), 'gm_mg_setup')
         if self.dense:
@@ -186,7 +190,8 @@ class Multigrid:
                                         L.ptr(self._Z), L.ptr(z), L.ptr(w), L.ptr(part), L.ptr(dh), m, tol,
                                         maxiter, L.ctypes.byref(it), L.ctypes.byref(rr), stream), 'gm_gmres_up')
             return it.value, rr.value
-        L.check(L.lib().gm_gmres_mg(self.dim, self.nb, int(self.fp32), self.n_levels, self.levels, self.nu_pre,
+        L.check(L.lib().gm_gmres_mg(self.dim, self.nb, int(self.fp32), self.n_levels, self.levels,
+                                    L.ptr(getattr(self, 'op_vals', None)), self.nu_pre,
                                     self.nu_post, self.coarse_sweeps, self.omega, L.ptr(b), L.ptr(x), L.ptr(V),
                                     L.ptr(self._Z), L.ptr(z), L.ptr(w), L.ptr(part), L.ptr(dh), m, tol, maxiter,
                                     L.ctypes.byref(it), L.ctypes.byref(rr), stream), 'gm_gmres_mg')

@@ -16,6 +16,7 @@ from ._base import _QuasiStaticBase
 class quasi_static_coupled_solver_2d(_QuasiStaticBase):
     dim = 2
     coupled = True
+    _STATE_EXTRA = ('particle_pressure_array',)
 
     def __init__(self, background_grid_dict, particles_dict, iteration_dict, material_dict, porous_media_dict,
                  load_dict, set_boundary_dofs, device=None, x_particles=None):

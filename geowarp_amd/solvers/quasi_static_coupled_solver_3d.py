@@ -25,6 +25,7 @@ def get_indentation_loss(indentation_force, footing_incr_disp, total_steps, targ
 class quasi_static_coupled_solver_3d(_QuasiStaticBase):
     dim = 3
     coupled = True
+    _STATE_EXTRA = ('particle_pressure_array',)
 
     def __init__(self, background_grid_dict, particles_dict, iteration_dict, material_dict, porous_media_dict,
                  load_dict, set_boundary_dofs, device=None, x_particles=None):
@@ -49,7 +50,20 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
         self.footing_initial_height = l['footing_initial_height']
         self.footing_incr_disp = l['footing_incr_disp']
         self.penalty_factor = l['penalty_factor']
+        # 'reference': r_z += -(h - (z + du_z)) k w_I (mpm_utils.py:1240-1246), a force pushing a
+        # penetrating particle up and a negative stiffness -k w_I w_J in the Newton matrix;
+        # 'physical': the textbook penalty r_z += (h - (z + du_z)) k w_I (push down, stiffness +k)
+        self.penalty_form = l.get('penalty_form', 'reference')
+        if self.penalty_form not in ('reference', 'physical'):
+            raise ValueError("penalty_form must be 'reference' or 'physical'")
         self.set_boundary_dofs = set_boundary_dofs
+        # multigrid hierarchy from the Newton matrix with the indenter penalty's sign reversed
+        # (the Krylov operator stays the exact matrix); False builds it from the matrix itself
+        self.engine.penalty_pc = iteration_dict.get('penalty_pc', True)
+        # adaptive load-step splitting on Newton failure (0: off, the reference's behaviour)
+        self.max_substeps = int(iteration_dict.get('substeps', 0))
+        self.substep_count = 0
+        self.step_newton_iterations = 0
         dev = self.device
         n = self.n_particles
         self.particle_pressure_array = torch.zeros(n, dtype=torch.float64, device=dev)
@@ -121,14 +135,17 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
         a.footing[2] = self.footing_region_min_y
         a.footing[3] = self.footing_region_max_y
         a.footing_height = self.footing_initial_height + (float(current_step) + 1.0) * self.footing_incr_disp
-        a.pen_par = self.youngs_modulus * self.penalty_factor * (self.dx / self.PPD) * (self.dy / self.PPD)
+        sign = 1.0 if self.penalty_form == 'reference' else -1.0
+        a.pen_par = sign * self.youngs_modulus * self.penalty_factor * (self.dx / self.PPD) * (self.dy / self.PPD)
         a.penalty = 1
         return a
 
-    def newton_iter(self, current_step, total_steps):
-        """qc3d:258-392 (Newton loop, then G2P and the lp update)."""
+    def newton_iter(self, current_step, total_steps, force_index=None):
+        """qc3d:258-392 (Newton loop, then G2P and the lp update).  force_index is
+        the indentation_force_array entry (default int(current_step); a sub-step of
+        load step s writes entry s)."""
         eng = self.engine
-        step = int(current_step)
+        step = int(current_step) if force_index is None else int(force_index)
         if step >= self.indentation_force_array.numel():
             grown = torch.zeros(step + 1, dtype=torch.float64, device=self.device)
             grown[:self.indentation_force_array.numel()] = self.indentation_force_array
@@ -147,8 +164,11 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
                 S = abs(self.footing_incr_disp) * float(total_steps)
                 self.E_grad = (2.0 * (F / S - TARGET_FORCE) * F / (self.youngs_modulus * S)).reshape(1).cpu().numpy()
 
-        self._newton(current_step, total_steps, self._new, self._old, self._args(current_step, total_steps),
-                     after_residual=after_residual)
+        args = self._args(current_step, total_steps)
+        eng.penalty = dict(footing=(self.footing_region_min_x, self.footing_region_max_x, self.footing_region_min_y,
+                                    self.footing_region_max_y), height=args.footing_height, pen_par=args.pen_par,
+                           lp_max=float(self.GIMP_lp.max()))
+        self._newton(current_step, total_steps, self._new, self._old, args, after_residual=after_residual)
         eng.g2p(self.x_particles, self.GIMP_lp, self._new, self._old, self.deformation_gradient_total_old,
                 self.deformation_gradient_total_new, self.left_Cauchy_Green_old, self.left_Cauchy_Green_new,
                 self.plastic_strain_old, self.plastic_strain_new, self.particle_pressure_array)
@@ -174,6 +194,40 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
         eng.set_nodal_pressure(xp, self._old, self._new, require_active=True)
 
     def advance_one_step(self, current_step, total_steps):
-        """qc3d:396-420."""
-        self.prepare_step(current_step, total_steps)
-        self.newton_iter(current_step, total_steps)
+        """qc3d:396-420.  With iteration_dict['substeps'] = d > 0 (not in the
+        reference, which moves on after n_iter iterations), a load step whose
+        Newton iteration does not converge is restarted from its initial state and
+        split into two halves, recursively down to 2^-d of the step: each part is a
+        full load step (P2G, Newton, G2P) to the load of its end point (the load
+        factor (s + f)/total and the indenter height h0 + (s + f) * incr are those
+        of a fractional current_step s - 1 + f) with the time step dt scaled by
+        the part's length.  substep_count and step_newton_iterations report the
+        parts and Newton iterations the step took."""
+        self.substep_count = 0
+        self.step_newton_iterations = 0
+        if not self.max_substeps:
+            self.prepare_step(current_step, total_steps)
+            self.newton_iter(current_step, total_steps)
+            self.substep_count, self.step_newton_iterations = 1, self.newton_iterations
+            return
+        s = float(current_step)
+        self._advance_range(s - 1.0, s, total_steps, 0, int(current_step))
+
+    def _advance_range(self, a, b, total_steps, depth, step_index):
+        snap = self.state_dict() if depth < self.max_substeps else None
+        dt_full = self.dt
+        self.dt = dt_full * (b - a)
+        try:
+            self.prepare_step(b, total_steps)
+            self.newton_iter(b, total_steps, force_index=step_index)
+        finally:
+            self.dt = dt_full
+        self.substep_count += 1
+        self.step_newton_iterations += self.newton_iterations
+        if self.newton_converged or snap is None:
+            return
+        self.load_state_dict(snap)
+        m = 0.5 * (a + b)
+        self._advance_range(a, m, total_steps, depth + 1, step_index)
+        if self.newton_converged:
+            self._advance_range(m, b, total_steps, depth + 1, step_index)

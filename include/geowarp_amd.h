@@ -86,6 +86,21 @@ int gm_node_residual(const gm_grid* g, const gm_bins* bins, const double* Y, con
 int gm_node_jacobian(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, const void* D, int tangent_fp32,
                      const uint8_t* freem, void* vals, int vals_fp32, int coupled, hipStream_t stream);
 
+/* Preconditioner copy of the 3D u-p Newton matrix: adds coef * w_I * w_J to
+   the zz entry of block (I, J) for every
+   particle the indenter penalises (z >= height inside the footing box
+   footing = {min_x, max_x, min_y, max_y}; predicate of mpm_utils.py:1242),
+   over the node rows of the box box_lo[a] .. box_lo[a] + box_cnt[a] - 1 and
+   free z-dofs only.  coef = 2 pen_par reverses the sign of the reference's
+   penalty term in K = -dR/du (mpm_utils.py:1240-1246, which makes K indefinite
+   under the indenter).  vals is a copy of the gm_node_jacobian output for
+   the multigrid hierarchy; the Krylov operator keeps K (gm_gmres_mg op_vals).
+   No reference counterpart (the reference solves with a direct or Jacobi-
+   BiCGStab solver, qc3d:335-363). */
+int gm_penalty_pc(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, const uint8_t* freem, void* vals,
+                  int vals_fp32, const int* box_lo, const int* box_cnt, const double* footing, double height,
+                  double coef, hipStream_t stream);
+
 /* P2G: activation flags (threshold < 0: every support node, P2G_2d
    mpm_utils.py:83-136; else w > threshold, P2G_coupled_2d/3d :209-223,
    :341-347), pressure-Dirichlet flags above flagged particles (:225-235,
@@ -252,14 +267,16 @@ int gm_mg_vcycle(int dim, int nb, int fp32, int n_levels, const gm_mg_level* lev
 int gm_mg_level_op(int nb, int fp32, const gm_mg_level* level, int level_index, int mode, const void* x,
                    const void* b, double omega, void* out, hipStream_t stream);
 
-/* Flexible GMRES(m) on level 0's matrix, right-preconditioned by the V-cycle;
-   arguments as gm_gmres (V: (m+1) * n vectors, Z: m * n preconditioned
-   directions, z, w: n; part >= 64(m+1) and >= 2048 doubles; dh >= 3(m+2)
-   doubles). */
-int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, int nu_pre, int nu_post,
-                int coarse_sweeps, double omega, const void* b, void* x, void* V, void* Z, void* z, void* w,
-                double* part, double* dh, int m, double tol, int maxiter, int* iters, double* relres,
-                hipStream_t stream);
+/* Flexible GMRES(m) right-preconditioned by the V-cycle.  The operator is
+   op_vals (in level 0's pattern and precision) when it is not null, else level
+   0's own matrix (op_vals lets the hierarchy be built from a modified copy of
+   the system, see gm_penalty_pc); other arguments as gm_gmres (V: (m+1) * n
+   vectors, Z: m * n preconditioned directions, z, w: n; part >= 64(m+1) and
+   >= 2048 doubles; dh >= 3(m+2) doubles). */
+int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_level* levels, const void* op_vals,
+                int nu_pre, int nu_post, int coarse_sweeps, double omega, const void* b, void* x, void* V, void* Z,
+                void* z, void* w, double* part, double* dh, int m, double tol, int maxiter, int* iters,
+                double* relres, hipStream_t stream);
 
 /* u-p systems (nb = dim + 1): block lower-triangular preconditioner
    z_u = MG(A_uu)^-1 v_u, z_p = S^-1 (v_p - K_pu z_u), S the diagonal Schur
