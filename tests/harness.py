@@ -15,11 +15,25 @@ MINI = {
     'bar_j2_platen': lambda: P.bar_j2(n_cells_x=4, n_cells_y=16, dx=0.0125, total_steps=10, tol=1e-9,
                                       platen_strain=0.02),
     'terzaghi': lambda: P.terzaghi(n_cells_x=2, n_cells_y=16, width=0.125, total_steps=10, dt=1.0),
+    # the full 2D Neo-Hookean bar of 8 x 80 cells (2560 particles), all 20 load steps
+    'c1_bar_nh': lambda: P.bar_nh(tol=1e-10),
+    # 16-cell column: the 10 kPa load undrained (dt 1e-5 s), then 7 log-spaced consolidation steps
+    'terzaghi_log': lambda: dict(P.terzaghi(n_cells_x=2, n_cells_y=16, width=0.125, total_steps=8, dt=1e-5),
+                                 dts=[1e-5] + list(np.diff(np.logspace(-3, np.log10(0.15), 8)))),
     'indent_nh': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.05, footing=0.5,
                                     material='Neo-Hookean', tol=1e-10),
     'cantilever': lambda: P.cantilever(PPD=2, total_steps=50, tol=1e-10),
+    # NorSand (the reference driver's material) in a rigid box under a platen, 4 x 8 cells
+    'bar_norsand': lambda: P.norsand_box(total_steps=10, strain=0.02, tol=1e-10),
+    # Drucker-Prager phi 30, c 5, psi 0 (the 3D indentation's material): the hyperbolic surface
+    # (shape factor 0.1) with its tension cut-off under the reference's penalty, and the cone with
+    # its apex return under the textbook penalty; the indenter starts at the top particle row
     'indent_dp': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.01, footing=0.5,
-                                    material='Drucker-Prager', tol=1e-10, n_iter=20),
+                                    material='Drucker-Prager', tol=1e-10, n_iter=20, dp_apex='return',
+                                    shape_factor=0.1, footing_start='surface'),
+    'indent_dp_cone': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.01, footing=0.5,
+                                         material='Drucker-Prager', tol=1e-10, n_iter=20, dp_apex='return',
+                                         footing_start='surface', penalty_form='physical'),
 }
 
 

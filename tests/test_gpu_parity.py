@@ -14,7 +14,8 @@ import pytest
 import harness as H
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
-CASES = ['bar_nh', 'bar_hencky', 'bar_j2', 'bar_dp', 'bar_j2_platen', 'terzaghi', 'indent_nh', 'cantilever']
+CASES = ['bar_nh', 'bar_hencky', 'bar_j2', 'bar_dp', 'bar_j2_platen', 'terzaghi', 'indent_nh', 'cantilever',
+         'indent_dp', 'indent_dp_cone', 'c1_bar_nh', 'terzaghi_log', 'bar_norsand']
 FIELD_TOL = 1e-8
 
 
@@ -32,6 +33,8 @@ def test_first_jacobian_matches_fd(name):
     cfg = H.MINI[name]()
     s = H.make(gs, cfg)
     total = cfg['total_steps']
+    if cfg.get('dts') is not None:
+        s.dt = cfg['dts'][0]
     s.prepare_step(0, total)
     u_new = s._inc if hasattr(s, '_inc') else s._new
     args = s._args(0, total)
@@ -61,6 +64,8 @@ def test_load_steps_match_oracle(name):
     s = H.make(gs, cfg)
     total = int(g['total_steps'][0])
     for k in range(int(g['n_steps'][0])):
+        if cfg.get('dts') is not None:
+            s.dt = cfg['dts'][k]
         s.advance_one_step(k, total)
         f = H.fields(s)
         print(name, 'step', k, 'newton', ['%.2e' % v for v in s.residual_history],

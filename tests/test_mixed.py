@@ -15,7 +15,8 @@ import pytest
 import harness as H
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
-CASES = ['bar_nh', 'bar_j2', 'bar_j2_platen', 'bar_dp', 'terzaghi', 'indent_nh']
+CASES = ['bar_nh', 'bar_j2', 'bar_j2_platen', 'bar_dp', 'terzaghi', 'indent_nh', 'indent_dp', 'indent_dp_cone',
+         'cantilever', 'terzaghi_log', 'bar_norsand']
 
 
 def _run_against_fixture(name, extra, engine_setup=None):
@@ -29,6 +30,8 @@ def _run_against_fixture(name, extra, engine_setup=None):
     total = int(g['total_steps'][0])
     x0 = s.x0.cpu().numpy()
     for k in range(int(g['n_steps'][0])):
+        if cfg.get('dts') is not None:
+            s.dt = cfg['dts'][k]
         s.advance_one_step(k, total)
         f = H.fields(s)
         print(name, 'step', k, 'newton', ['%.2e' % v for v in s.residual_history],
