@@ -121,13 +121,15 @@ def bar_j2(n_cells_x=256, n_cells_y=1024, dx=1.953125e-3, total_steps=50, tol=1e
 
 
 def norsand_box(n_cells_x=4, n_cells_y=8, dx=0.0125, total_steps=10, strain=0.02, tol=1e-10, precision='fp64',
-                norsand=None):
+                norsand=None, prestrain=-0.002):
     """2D plane-strain NorSand specimen in a rigid box (x fixed on the side walls,
     fixed base) compressed by a top platen by `strain` of its height
     (oedometric loading), no gravity.  The particles start at the isotropic
-    NorSand pressure p0 at zero strain, in equilibrium with the walls; the
-    material is the one of the reference's NorSand driver
-    (triaxial_NorSand.py:8-30) unless `norsand` overrides keys."""
+    NorSand pressure p0, in equilibrium with the walls, at the elastic
+    volumetric strain `prestrain` (b_old = exp(2 prestrain / 3) I and ev0 =
+    prestrain, so that the return's tension cut at zero volumetric strain is
+    |prestrain| away); the material is the one of the reference's NorSand
+    driver (triaxial_NorSand.py:8-30) unless `norsand` overrides keys."""
     g = _grid2d(n_cells_x, n_cells_y, dx)
     PPD = 2
     npart = n_cells_x * n_cells_y * PPD * PPD
@@ -137,10 +139,11 @@ def norsand_box(n_cells_x=4, n_cells_y=8, dx=0.0125, total_steps=10, strain=0.02
         particles_dict=dict(n_particles=npart, PPD=PPD, p_vol=(dx / PPD) ** 2, p_rho=2.0),
         iteration_dict=dict(n_iter=12, tol=tol, tol_mode='relative', precision=precision),
         material_dict=dict(youngs_modulus=1.0e5, poisson_ratio=0.3, material_name='NorSand',
-                           norsand=dict(norsand or {})),
+                           norsand=dict(dict(ev0=prestrain), **(norsand or {}))),
         load_dict=dict(gravity_mag=0.0, traction_value_x=0.0, traction_value_y=0.0, point_load_value_x=0.0,
                        point_load_value_y=0.0, prescribed_increment=make_platen(-strain * n_cells_y * dx)),
         set_boundary_dofs=make_bar_bc(1, 1 + n_cells_x, 1),
+        initial_volumetric_strain=prestrain,
         total_steps=total_steps,
     )
 
@@ -366,6 +369,11 @@ def build_solver(module, cfg, **kw):
     else:
         s = cls(cfg['background_grid_dict'], cfg['particles_dict'], cfg['iteration_dict'], cfg['material_dict'],
                 cfg['porous_media_dict'], cfg['load_dict'], cfg['set_boundary_dofs'], **kw)
+    ev0 = cfg.get('initial_volumetric_strain')
+    if ev0:
+        # isotropic elastic pre-strain: b_old = exp(2 ev0 / 3) b_old, F_old unchanged
+        b = s.b_old if hasattr(s, 'b_old') else s.left_Cauchy_Green_old
+        b *= np.exp(2.0 * ev0 / 3.0)
     return s
 
 

@@ -98,6 +98,8 @@ def fd_jacobian(resfun, u, n_grid, dim, nb, n_nodes, boundary, act, steps):
 
 class _NewtonMixin:
     tol_mode = 'absolute'
+    # central-difference step of the coloured Jacobian, relative to dx (displacement dofs)
+    fd_rel = 1e-6
 
     def first_jacobian(self, current_step, total_steps):
         """Reference-convention matrix (-dR/du, masked, identity rows) and
@@ -211,7 +213,7 @@ class quasi_static_solver_2d(_NewtonMixin):
             self._prescribed = pres
         st = og.Stencil(self.x_particles, self.lp, 2, (self.dx,), self.n_grid)
         self.act = om.p2g_2d_activate(st, self.n_nodes)
-        steps = (1e-6 * self.dx, 1e-6 * self.dx)
+        steps = (self.fd_rel * self.dx, self.fd_rel * self.dx)
         return st, self._res(st, current_step, total_steps), self.increment_solution.copy(), steps
 
     def advance_one_step(self, current_step, total_steps):
@@ -325,7 +327,7 @@ class quasi_static_coupled_solver_2d(_NewtonMixin):
         xp = om.solve_consistent_mass(M, rhs)
         self.old_solution, self.new_solution = om.p2g_set_nodal_solution(
             self.boundary, self.act, self.n_nodes, 2, self.old_solution, self.new_solution, xp, False)
-        steps = (1e-6 * self.dx, 1e-6 * self.dx, 1.0)
+        steps = (self.fd_rel * self.dx, self.fd_rel * self.dx, 1.0)
         return st, self._res(st, current_step, total_steps), self.new_solution.copy(), steps
 
     def advance_one_step(self, current_step, total_steps):
@@ -439,7 +441,7 @@ class quasi_static_coupled_solver_3d(_NewtonMixin):
         xp = om.solve_consistent_mass(M, rhs)
         self.old_solution, self.new_solution = om.p2g_set_nodal_solution(
             self.boundary, self.act, self.n_nodes, 3, self.old_solution, self.new_solution, xp, True)
-        steps = (1e-6 * self.dx, 1e-6 * self.dx, 1e-6 * self.dx, 1.0)
+        steps = (self.fd_rel * self.dx, self.fd_rel * self.dx, self.fd_rel * self.dx, 1.0)
         return st, self._res(st, current_step, total_steps), self.new_solution.copy(), steps
 
     def advance_one_step(self, current_step, total_steps):

@@ -23,16 +23,20 @@ MINI = {
     'indent_nh': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.05, footing=0.5,
                                     material='Neo-Hookean', tol=1e-10),
     'cantilever': lambda: P.cantilever(PPD=2, total_steps=50, tol=1e-10),
-    # NorSand (the reference driver's material) in a rigid box under a platen, 4 x 8 cells
-    'bar_norsand': lambda: P.norsand_box(total_steps=10, strain=0.02, tol=1e-10),
+    # NorSand (the reference driver's material) in a rigid box under a platen, 4 x 8 cells.  The
+    # platen moves 0.1 % of the height per step (the first iterate strains the top cells by 0.8 %,
+    # where the fixed 15-iteration local Newton still converges, so its tangent is the derivative
+    # of the stress); the particles start pre-strained by -0.2 % volumetric, away from the
+    # return's tension cut at zero volumetric strain.
+    'bar_norsand': lambda: P.norsand_box(total_steps=10, strain=0.01, tol=1e-10),
     # Drucker-Prager phi 30, c 5, psi 0 (the 3D indentation's material): the hyperbolic surface
     # (shape factor 0.1) with its tension cut-off under the reference's penalty, and the cone with
     # its apex return under the textbook penalty; the indenter starts at the top particle row
     'indent_dp': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.01, footing=0.5,
-                                    material='Drucker-Prager', tol=1e-10, n_iter=20, dp_apex='return',
+                                    material='Drucker-Prager', tol=1e-12, n_iter=20, dp_apex='return',
                                     shape_factor=0.1, footing_start='surface'),
     'indent_dp_cone': lambda: P.indent3d(n_cells=(4, 4, 4), dx=0.25, total_steps=10, depth=0.01, footing=0.5,
-                                         material='Drucker-Prager', tol=1e-10, n_iter=20, dp_apex='return',
+                                         material='Drucker-Prager', tol=1e-12, n_iter=20, dp_apex='return',
                                          footing_start='surface', penalty_form='physical'),
 }
 
@@ -61,6 +65,8 @@ def make(module, cfg, **kw):
     """Build the solver and apply the workload's boundary / particle flags."""
     s = P.build_solver(module, cfg, **kw)
     is_oracle = module.__name__.startswith('oracle')
+    if is_oracle and 'fd_rel' in cfg:
+        s.fd_rel = cfg['fd_rel']
     if cfg['solver'] == 'quasi_static_solver_2d':
         flags = s.boundary if is_oracle else s.dofStruct.boundary_flag_array
         cfg['set_boundary_dofs'](flags, s.n_grid_x, s.n_nodes)
