@@ -23,5 +23,15 @@ Parity pins (what checks each part of the oracle, see tests/test_oracle_*.py):
 * Return mappings: yield-surface consistency of the converged state, J2 closed
   form, Drucker-Prager cone closed form (shape factor 0), NorSand local
   residual equations to round-off.
+* Coupled u-p terms (tests/test_oracle_coupled.py): consistent-mass rows and
+  linear-pressure recovery, the pressure block's symmetry and null space, the
+  Darcy term of a linear pressure equal to the boundary flux, the undrained
+  column carrying the load in pore pressure, Terzaghi's series, the penalty
+  indenter force in closed form.
 * Linear solves: scipy.sparse.linalg.spsolve.
+
+cpu_port.cpp / cpu_port.py restate the 3D u-p Neo-Hookean assembly in C++17 with
+OpenMP (the benchmark's CPU baseline).  Its Jacobian is exact, by forward-mode
+dual numbers, and is checked against this package's residual and coloured
+central differences (tests/test_cpu_port.py).
 """
