@@ -88,6 +88,10 @@ _SIGS = {
     'gm_node_residual': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), _P, _P, _P, c_int, _P]),
     'gm_node_jacobian': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern), _P,
                                  c_int, _P, _P, c_int, c_int, _P]),
+    'gm_node_jacobian_work_bytes': (ctypes.c_size_t, [ctypes.POINTER(gm_grid), c_int]),
+    'gm_node_jacobian_binned': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins),
+                                        ctypes.POINTER(gm_pattern), _P, c_int, _P, _P, c_int, _P, ctypes.c_size_t,
+                                        _P]),
     'gm_p2g': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern),
                        c_double, _P, c_double, c_double, _P, _P, _P, _P, _P, _P]),
     'gm_free_mask': (c_int, [c_int, _P, _P, _P, _P]),

@@ -102,6 +102,10 @@ class Engine:
         self.penalty = None
         self.penalty_pc = True
         self.vals_pc = None
+        # node Jacobian of the 3D u-p form: 'bins' (per-bin products on the matrix cores into a
+        # workspace of n_nodes x 11664 values, then a fixed-order gather) or 'rows' (one
+        # workgroup per block row, no workspace)
+        self.jacobian_kernel = os.environ.get('GM_JACOBIAN_KERNEL', 'bins')
 
     # ------------------------------------------------------------------ buffers
     def _alloc_vals(self):
@@ -294,16 +298,26 @@ class Engine:
             # FP64 assembly, FP32 copy for the Krylov solve
             if self.vals64 is None or self.vals64.numel() != self.vals.numel():
                 self.vals64 = torch.zeros(self.vals.numel(), dtype=torch.float64, device=self.dev)
-            L.check(L.lib().gm_node_jacobian(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins),
-                                             L.ctypes.byref(self.pattern), L.ptr(self.D), 0, L.ptr(self.freem),
-                                             L.ptr(self.vals64), 0, int(self.coupled), _stream()),
-                    'gm_node_jacobian')
+            self._assemble(False, self.vals64, False)
             L.check(L.lib().gm_d2f(self.nnzb * self.nb * self.nb, L.ptr(self.vals64), L.ptr(self.vals), _stream()),
                     'gm_d2f')
             return
-        L.check(L.lib().gm_node_jacobian(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins),
-                                         L.ctypes.byref(self.pattern), L.ptr(self.D), int(fp32), L.ptr(self.freem),
-                                         L.ptr(self.vals), int(fp32), int(self.coupled), _stream()),
+        self._assemble(fp32, self.vals, fp32)
+
+    def _assemble(self, tangent_fp32, vals, vals_fp32):
+        g, b, pat = L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ctypes.byref(self.pattern)
+        if self.dim == 3 and self.coupled and self.jacobian_kernel == 'bins':
+            need = int(L.lib().gm_node_jacobian_work_bytes(g, int(tangent_fp32)))
+            w = self._ws.get('jacobian_bins')
+            if w is None or w.numel() < need:
+                self._ws.pop('jacobian_bins', None)
+                w = self._ws['jacobian_bins'] = torch.empty(need, dtype=torch.uint8, device=self.dev)
+            L.check(L.lib().gm_node_jacobian_binned(g, b, pat, L.ptr(self.D), int(tangent_fp32), L.ptr(self.freem),
+                                                    L.ptr(vals), int(vals_fp32), L.ptr(w), need, _stream()),
+                    'gm_node_jacobian_binned')
+            return
+        L.check(L.lib().gm_node_jacobian(g, b, pat, L.ptr(self.D), int(tangent_fp32), L.ptr(self.freem), L.ptr(vals),
+                                         int(vals_fp32), int(self.coupled), _stream()),
                 'gm_node_jacobian')
 
     def gmres_workspace(self, nb, fp32, m):

@@ -86,6 +86,18 @@ int gm_node_residual(const gm_grid* g, const gm_bins* bins, const double* Y, con
 int gm_node_jacobian(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, const void* D, int tangent_fp32,
                      const uint8_t* freem, void* vals, int vals_fp32, int coupled, hipStream_t stream);
 
+/* The same matrix for the 3D u-p form (the reference's qc3d:285-321 coloured
+   reverse sweeps), assembled per particle bin on the matrix cores: each bin's
+   27 x 27 node blocks go to the workspace `work` (at least
+   gm_node_jacobian_work_bytes(g, tangent_fp32) bytes: n_nodes x 11664 values
+   of the tangent's precision), then every block of the pattern sums the bins
+   that hold both of its nodes in a fixed order.  Arithmetic in the tangent's
+   precision; -1 for another form or a smaller workspace. */
+size_t gm_node_jacobian_work_bytes(const gm_grid* g, int tangent_fp32);
+int gm_node_jacobian_binned(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, const void* D,
+                            int tangent_fp32, const uint8_t* freem, void* vals, int vals_fp32, void* work,
+                            size_t work_bytes, hipStream_t stream);
+
 /* Preconditioner copy of the 3D u-p Newton matrix: adds coef * w_I * w_J to
    the zz entry of block (I, J) for every
    particle the indenter penalises (z >= height inside the footing box
