@@ -2,35 +2,39 @@
 
 Workload (default): 3D u-p indentation, soil block 1 x 1 x 0.5 m on 64 x 64 x 32
 cells (dx = 1/64), 2x2x2 particles per cell = 1,048,576 particles, 4 dofs per
-node, E 20 MPa, nu 0.3, k/mu_f 1e-6, rigid 0.2 x 0.2 m corner indenter pushed
-0.05 m in 40 load steps.  Material: Neo-Hookean by default (--material
-Drucker-Prager selects phi 30, c 5 kPa, psi 0; its Newton iterations do not
-converge on this workload with the present linear solver).  A "step" is one
-load step converged to 1e-8 relative Newton residual (P2G + consistent-mass
-projection + Newton: particle stage, residual, Jacobian, multigrid-
-preconditioned flexible GMRES, backtracking update + G2P); the timed step is
-load step 1 after load step 0 as warm-up.
+node, E 20 MPa, nu 0.3, k/mu_f 1e-6, no gravity, rigid 0.2 x 0.2 m corner
+indenter pushed 0.05 m in 40 load steps, starting at the top particle row, with
+the textbook penalty sign.  Material: Neo-Hookean (--material Drucker-Prager
+selects phi 30, c 5 kPa, psi 0: on this stress-free block the indenter drives
+about a fifth of the particles past the tension capacity and the Newton
+iterations stall, so that run reports its failed load step and no value).
+A "step" is one load step converged to 1e-8 relative Newton residual (P2G +
+consistent-mass projection + Newton: particle stage, residual, Jacobian,
+multigrid-preconditioned flexible GMRES, backtracking update + G2P; a load step
+whose Newton iteration fails is halved up to --substeps times).  --warmup load
+steps run untimed, then --steps load steps are timed (defaults 5 and 20: load
+steps 5-24).
 
-The timed form is the mixed-precision solver (FP32 Jacobian and Krylov, FP64
-residual); the plain FP64 form runs the same steps from the same state and
-both are reported, with the mixed/FP64 speed ratio and field errors.
-value = Newton iterations per second of the mixed form over the timed steps.
+The timed form is the mixed-precision solver (FP32 Jacobian and Krylov, FP16
+smoother copies, FP64 residual); the plain FP64 form runs the same load steps
+from the same state (as many as the --budget allows) and both are reported,
+with the mixed/FP64 time ratio over the compared steps and the field errors at
+the last of them.  value = Newton iterations per second of the mixed form over
+the timed load steps; every timed step must converge, or value is null.
 
-roofline: the dominant kernel of the timed form, the level-0 multigrid smoother
-sweep (k_level from the FP16 copy: x' = x + omega Dinv (b - A x)), timed live with
-HIP events on the launch stream over repeated launches on the hierarchy of the last
-Jacobian of the timed region; algorithmic bytes per launch = nnzb*(2 b^2 + 4) +
-4*(n_nodes+1) + 4 b^2 n_nodes (scales) + s b^2 n_nodes (Dinv) + 3*n*s.
-roofline_krylov_spmv: the FP32 block-CSR SpMV of the Krylov operator, likewise;
-algorithmic bytes per launch = nnzb*(b^2*s + 4) + 4*(n_nodes+1) + 2*n*s.
-cpu_baseline: the oracle (NumPy, one thread) on a bounded sample: one Newton
-iteration (residual + coloured-FD Jacobian + sparse direct solve) of the same
-workload restricted to 8 x 8 x 4 cells, scaled to the full particle count.
+roofline: the dominant kernel of the timed form, timed live with HIP events on
+its launch stream over repeated launches on the last Jacobian of the timed
+region, with its algorithmic bytes per launch (stated in the entry); traffic is
+the counted HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+WRITE_SIZE passes (profiles/), or null.  roofline_krylov_spmv: the FP32
+block-CSR SpMV of the Krylov operator, likewise.
+cpu_baseline: one Newton iteration of the same workload's first load step on its
+16 x 16 x 8-cell sub-block, on the host: the C++/OpenMP FP64 restatement of the
+assembly (oracle/cpu_port.cpp: residual + exact Jacobian, every core this
+process may use) and a SciPy SuperLU solve, scaled linearly in particle count.
 
-Further blocks of the same JSON line (each skippable by its option): the same
-steps in the mixed form with Newton run to 1e-11 (field errors set by precision,
-not by the tolerance); c2_j2_bar, the 2D J2 bar of 256 x 1024 cells (FP64 and
-mixed); c3_consolidation, the 2D consolidation column against Terzaghi's series;
+--extras adds: c2_j2_bar, the 2D J2 bar of 256 x 1024 cells (FP64 and mixed);
+c3_consolidation, the 2D consolidation column against Terzaghi's series;
 adjoint, the batched 64-specimen inverse-analysis gradient with a central
 finite-difference check of one component.
 """
@@ -53,31 +57,46 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=1, help='timed load steps')
-    ap.add_argument('--warmup', type=int, default=1, help='untimed load steps before the timed ones')
+    ap.add_argument('--steps', type=int, default=20, help='timed load steps')
+    ap.add_argument('--warmup', type=int, default=5, help='untimed load steps before the timed ones')
     ap.add_argument('--cells', type=str, default='64,64,32')
     ap.add_argument('--material', type=str, default='Neo-Hookean')
-    ap.add_argument('--dp-apex', type=str, default='reference',
-                    help="Drucker-Prager apex rule: 'reference' (no return for P_trial >= 0) or 'return'")
+    ap.add_argument('--dp-apex', type=str, default='return',
+                    help="Drucker-Prager apex rule: 'return' (apex return) or 'reference' (no return for P_trial >= 0)")
+    ap.add_argument('--shape-factor', type=float, default=0.1,
+                    help='Drucker-Prager hyperbolic shape factor (0: the cone with an apex)')
+    ap.add_argument('--footing-start', type=str, default='surface',
+                    help="'surface': the indenter starts at the top particle row (every step pushes one "
+                         "increment); 'reference': end_z - dz/PPD as the reference driver")
+    ap.add_argument('--penalty-form', type=str, default='physical',
+                    help="indenter penalty: 'physical' (pushes penetrating soil down) or 'reference' (the "
+                         "reference's sign, a negative stiffness under the indenter)")
+    ap.add_argument('--substeps', type=int, default=4, help='levels of load-step halving on Newton failure')
+    ap.add_argument('--budget', type=float, default=540.0,
+                    help='wall-clock seconds for the whole run; the optional blocks are skipped past it')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--verbose', action='store_true')
+    ap.add_argument('--extras', action='store_true',
+                    help='also run the 2D J2 bar, the consolidation column and the batched adjoint blocks')
     ap.add_argument('--adjoint-steps', type=int, default=30,
-                    help='load steps of the batched-specimen adjoint measurement (0 = skip)')
+                    help='load steps of the batched-specimen adjoint measurement (with --extras; 0 = skip)')
     ap.add_argument('--krylov', type=str, default='mg', help="'mg' (multigrid GMRES) or 'gmres' (block Jacobi)")
-    ap.add_argument('--c2-steps', type=int, default=2, help='load steps of the 2D J2 bar measurement (0 = skip)')
+    ap.add_argument('--c2-steps', type=int, default=2, help='load steps of the 2D J2 bar (with --extras; 0 = skip)')
     ap.add_argument('--c3-steps', type=int, default=101,
-                    help='load steps of the 2D consolidation column measurement (0 = skip)')
+                    help='load steps of the 2D consolidation column (with --extras; 0 = skip)')
     return ap.parse_args()
 
 
-def build_solver(cells, precision, material, tol=1e-8, **kw):
+def build_solver(cells, precision, args, tol=1e-8):
     import geowarp_amd.solvers as gs
     from geowarp_amd import problems as P
     dx = 1.0 / 64
-    kw.setdefault('n_iter', 12)
     cfg = P.indent3d(n_cells=cells, dx=dx, total_steps=40, depth=0.05, footing=0.2 * cells[0] / 64.0,
-                     material=material, precision=precision, tol=tol, **kw)
+                     material=args.material, precision=precision, tol=tol, n_iter=12, krylov=args.krylov,
+                     dp_apex=args.dp_apex, shape_factor=args.shape_factor, footing_start=args.footing_start,
+                     penalty_form=args.penalty_form)
+    cfg['iteration_dict']['substeps'] = args.substeps
     s = P.build_solver(gs, cfg, device='cuda:0')
     cfg['particle_flags'](s.x_particles, s.particle_traction_flag_array, s.particle_pressure_boundary_flag_array)
     s.line_search = True
@@ -85,17 +104,45 @@ def build_solver(cells, precision, material, tol=1e-8, **kw):
     return s, cfg
 
 
-def run_steps(s, cfg, first, count, timer=None):
+def run_steps(s, cfg, first, count, log=None):
+    """Advance load steps first .. first+count-1; stop at the first step whose
+    Newton iteration (with its sub-steps) does not converge.  Returns per-step
+    records."""
     import torch
-    iters = 0
-    kry = 0
+    recs = []
     s.engine.stats['krylov_iters'].clear()
     for k in range(first, first + count):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n_kry = len(s.engine.stats['krylov_iters'])
         s.advance_one_step(k, cfg['total_steps'])
-        iters += s.newton_iterations
-    kry = sum(s.engine.stats['krylov_iters'])
-    torch.cuda.synchronize()
-    return iters, kry
+        torch.cuda.synchronize()
+        h = s.residual_history
+        rec = dict(step=k, seconds=round(time.perf_counter() - t0, 4), newton_iters=s.step_newton_iterations,
+                   substeps=s.substep_count, krylov_iters=int(sum(s.engine.stats['krylov_iters'][n_kry:])),
+                   converged=bool(s.newton_converged),
+                   relative_residual=(h[-1] / h[0]) if h and h[0] > 0 else None,
+                   indentation_force=float(s.indentation_force.item()))
+        recs.append(rec)
+        if log:
+            print(log, json.dumps(rec), file=sys.stderr, flush=True)
+        if not rec['converged']:
+            break
+    return recs
+
+
+def summarize(recs, count):
+    ok = len(recs) == count and all(r['converged'] for r in recs)
+    sec = sum(r['seconds'] for r in recs)
+    it = sum(r['newton_iters'] for r in recs)
+    out = dict(seconds=round(sec, 4), load_steps=len(recs), all_converged=ok, newton_iters=it,
+               krylov_iters=sum(r['krylov_iters'] for r in recs), substeps=sum(r['substeps'] for r in recs),
+               newton_per_s=round(it / sec, 4) if sec > 0 else None,
+               ms_per_load_step=round(1e3 * sec / max(len(recs), 1), 3))
+    if not ok and recs:
+        bad = recs[-1]
+        out['failed_step'] = dict(step=bad['step'], relative_residual=bad['relative_residual'])
+    return out
 
 
 def spmv_roofline(s, reps=50):
@@ -350,7 +397,7 @@ def c3_bench(steps):
         solvers[form] = s
     a, b = solvers['mixed'], solvers['fp64']
     out['mixed_vs_fp64_speedup'] = round(out['fp64']['seconds'] / out['mixed']['seconds'], 3)
-    out['mixed_vs_fp64_field_rel_err'] = field_errors(a, b)
+    out['mixed_vs_fp64_field_rel_err'] = field_errors(fields(a), fields(b))
     out['fp64_vs_terzaghi'] = dict(undrained_pressure_kPa=p0, consolidation_coefficient=cv, checks=series)
     out['workload'] = ('2D u-p consolidation column 32x320 cells, %d particles, %d load steps (log-spaced times to '
                        't = %g s) to 1e-8 relative residual' % (b.n_particles, steps, t_end))
@@ -369,49 +416,70 @@ def measured_traffic(cells, material, kernel='k_spmv'):
         return json.load(f)['traffic_bytes_per_launch']
 
 
-def field_errors(a, b):
+def fields(s):
+    """Comparable end-of-step fields of a solver (device copies)."""
+    return dict(u=(s.x_particles - s.x0).clone(), sigma=s.particle_Cauchy_stress_array.clone(),
+                p=s.particle_pressure_array.clone())
+
+
+def field_errors(fa, fb):
+    """Relative L2 differences of fields fa against fb (dicts from fields())."""
     import torch
-    out = {}
-    ua = (a.x_particles - a.x0)
-    ub = (b.x_particles - b.x0)
-    out['displacement'] = float(torch.linalg.norm(ua - ub) / torch.linalg.norm(ub))
-    out['stress'] = float(torch.linalg.norm(a.particle_Cauchy_stress_array - b.particle_Cauchy_stress_array)
-                          / torch.linalg.norm(b.particle_Cauchy_stress_array))
-    pb = torch.linalg.norm(b.particle_pressure_array)
-    out['pore_pressure'] = float(torch.linalg.norm(a.particle_pressure_array - b.particle_pressure_array) / pb) \
-        if pb > 0 else 0.0
-    return out
+
+    def rel(x, y):
+        ny = torch.linalg.norm(y)
+        return float(torch.linalg.norm(x - y) / ny) if ny > 0 else float(torch.linalg.norm(x))
+    return dict(displacement=rel(fa['u'], fb['u']), stress=rel(fa['sigma'], fb['sigma']),
+                pore_pressure=rel(fa['p'], fb['p']))
 
 
-def cpu_baseline():
-    """Oracle Newton iteration on an 8 x 8 x 4-cell sample of the workload."""
-    try:
-        from threadpoolctl import threadpool_limits
-    except Exception:  # pragma: no cover
-        threadpool_limits = None
+def cpu_baseline(args):
+    """One Newton iteration of the timed workload's first load step on its 16 x 16 x 8-cell
+    corner sub-block (same material, indenter and penalty), on the host: the C++/OpenMP
+    restatement of the assembly (residual + exact Jacobian, oracle/cpu_port.cpp) on every
+    core this process may use, then a sparse direct solve (SciPy SuperLU, the reference's
+    host solver family), median of three; scaled linearly in particle count."""
+    if args.material != 'Neo-Hookean':
+        return None
     sys.path.insert(0, os.path.join(ROOT, 'tests'))
     import harness as H
     import oracle.solvers as osv
-    from geowarp_amd import problems as P
-    cells = (8, 8, 4)
-    cfg = P.indent3d(n_cells=cells, dx=1.0 / 64, total_steps=40, depth=0.05, footing=0.2 * 8 / 64.0,
-                     material='Drucker-Prager', tol=1e-8)
-    s = H.make(osv, cfg)
-    ctx = threadpool_limits(1) if threadpool_limits else None
-    t0 = time.perf_counter()
-    K, r, u0 = s.first_jacobian(0, cfg['total_steps'])
+    import oracle.cpu_port as cp
     from scipy.sparse.linalg import spsolve
-    spsolve(K.tocsc(), r)
-    t = time.perf_counter() - t0
-    if ctx is not None:
-        ctx.unregister() if hasattr(ctx, 'unregister') else None
-    n_sample = s.n_particles
+    from geowarp_amd import problems as P
+    cells = (16, 16, 8)
+    cfg = P.indent3d(n_cells=cells, dx=1.0 / 64, total_steps=40, depth=0.05, footing=0.2 * cells[0] / 64.0,
+                     material=args.material, tol=1e-8, footing_start=args.footing_start,
+                     penalty_form=args.penalty_form)
+    s = H.make(osv, cfg)
+    st, f, u0, _ = s.prepare_step(0, cfg['total_steps'])
+    A = cp.Assembler(s, st, 0, cfg['total_steps'])
+    ts, ta = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        r = A.assemble(u0)
+        K = A.matrix()
+        t1 = time.perf_counter()
+        spsolve(K.tocsc(), r)
+        ts.append(time.perf_counter() - t0)
+        ta.append(t1 - t0)
+    t = sorted(ts)[1]
     n_full = 64 * 64 * 32 * 8
-    t_full = t * n_full / n_sample
-    return dict(value=1.0 / t_full, unit='Newton steps/s', cores=1, kind='port',
-                sample='oracle (NumPy FP64, 1 thread) Newton iteration = residual + coloured-FD Jacobian + '
-                       'spsolve on the 8x8x4-cell sub-block (%d particles) of the same workload, %.1f s, scaled '
-                       'linearly to %d particles' % (n_sample, t, n_full))
+    t_full = t * n_full / s.n_particles
+    model = ''
+    try:
+        with open('/proc/cpuinfo') as fh:
+            model = next((ln.split(':', 1)[1].strip() for ln in fh if ln.startswith('model name')), '')
+    except OSError:
+        pass
+    return dict(value=round(1.0 / t_full, 6), unit='Newton steps/s', cores=A.n_threads, kind='port',
+                cpu=model,
+                sample='C++/OpenMP FP64 restatement (oracle/cpu_port.cpp, %d threads) of one Newton iteration: '
+                       'residual + exact Jacobian (%.3f s) + SciPy SuperLU solve (1 thread), median of 3 = %.3f s '
+                       'on the 16x16x8-cell sub-block (%d particles, %d dofs) of the same workload, scaled '
+                       'linearly to %d particles (the direct solve grows faster than linearly, so the scaled '
+                       'figure flatters the CPU)' % (A.n_threads, sorted(ta)[1], t, s.n_particles, 4 * A.nn,
+                                                      n_full))
 
 
 def main():
@@ -420,94 +488,116 @@ def main():
         print('bench.py measures one GPU only: the path is single-device (multi-GPU sharding is a design note, '
               'not built); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
+    t_start = time.perf_counter()
     import torch
-    from geowarp_amd.solvers._engine import PhaseTimer
     cells = tuple(int(c) for c in args.cells.split(','))
     torch.cuda.set_device(0)
+    log = 'bench' if args.verbose else None
 
-    res = {}
-    forms = ['mixed'] if args.no_fp64 else ['fp64', 'mixed']
-    solvers = {}
-    for form in forms:
-        s, cfg = build_solver(cells, form, args.material, krylov=args.krylov, dp_apex=args.dp_apex)
-        solvers[form] = s
-        run_steps(s, cfg, 0, args.warmup)
-        s.engine.timers = PhaseTimer()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        iters, kry = run_steps(s, cfg, args.warmup, args.steps)
-        t1 = time.perf_counter()
-        phases = s.engine.timers.summary()
-        s.engine.timers = None
-        res[form] = dict(seconds=t1 - t0, newton_iters=iters, krylov_iters=kry,
-                         spmv_autotune=s.engine.stats.get('spmv_tune'),
-                         ms_per_step=(t1 - t0) * 1e3 / args.steps, newton_per_s=iters / (t1 - t0),
-                         phases_ms={k: round(v, 3) for k, v in phases.items()},
-                         last_residuals=['%.2e' % v for v in s.residual_history])
-        if args.verbose:
-            print(form, json.dumps(res[form]), file=sys.stderr)
-    main_form = 'mixed'
-    roof = spmv_roofline(solvers[main_form])
-    roof64 = spmv_roofline(solvers['fp64']) if 'fp64' in solvers else None
-    smo = smoother_roofline(solvers[main_form])
-    out = {
-        'metric': METRIC,
-        'value': round(res[main_form]['newton_per_s'], 4),
-        'unit': 'Newton steps/s',
-        'n_gpus': 1,
-        'steps': args.steps,
-        'warmup': args.warmup,
-        'ms_per_step': round(res[main_form]['ms_per_step'], 3),
-        'higher_is_better': True,
-        'scaling': 'weak',
-        'vs_baseline': None,
-        'dtype': 'f32+f64 (FP32 Jacobian/Krylov, FP64 residual)',
-        'data': 'synthetic',
-        'config': {'workload': '3D u-p indentation %dx%dx%d cells, %d particles, %s, load steps %d-%d to 1e-8 '
-                               'relative residual' % (cells + (solvers[main_form].n_particles, args.material,
-                                                               args.warmup, args.warmup + args.steps - 1)),
-                   'nodes': solvers[main_form].n_nodes, 'dofs': solvers[main_form].n_matrix_size,
-                   'nnzb': solvers[main_form].engine.nnzb},
-        'mixed': res.get('mixed'),
-        'fp64': res.get('fp64'),
-        # the dominant kernel of the timed (mixed) form: each GMRES iteration runs three level-0
-        # smoother products (residual + two Jacobi sweeps, from the FP16 copy) and one FP32
-        # Krylov SpMV; both are reported, the smoother sweep as `roofline`
-        'roofline': {'bound': 'hbm', 'achieved': round(smo['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': round(smo['achieved_GBs'] / HBM_PEAK_GBS, 4),
-                     'traffic': measured_traffic(cells, args.material, 'k_level') if main_form == 'mixed' else None,
-                     'kernel': smo['kernel'], 'avg_us': round(smo['avg_us'], 2),
-                     'algorithmic_bytes': smo['bytes_per_launch']},
-        'roofline_krylov_spmv': {'bound': 'hbm', 'achieved': round(roof['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS,
-                                 'unit': 'GB/s', 'frac': round(roof['achieved_GBs'] / HBM_PEAK_GBS, 4),
-                                 'traffic': measured_traffic(cells, args.material),
-                                 'kernel': roof['kernel'], 'avg_us': round(roof['avg_us'], 2),
-                                 'algorithmic_bytes': roof['bytes_per_launch']},
-    }
-    if roof64:
-        out['roofline_fp64_spmv'] = {'achieved': round(roof64['achieved_GBs'], 1), 'frac': round(
-            roof64['achieved_GBs'] / HBM_PEAK_GBS, 4), 'avg_us': round(roof64['avg_us'], 2),
-            'kernel': roof64['kernel']}
-    if 'fp64' in res:
-        out['mixed_vs_fp64_speedup'] = round(res['fp64']['seconds'] / res['mixed']['seconds'], 3)
-        out['mixed_vs_fp64_field_rel_err'] = field_errors(solvers['mixed'], solvers['fp64'])
-        # the same load steps in the mixed form with Newton run to 1e-11 (untimed): the field
-        # differences above are set by the 1e-8 Newton tolerance, these by the precision
-        st, cfgt = build_solver(cells, 'mixed', args.material, tol=1e-11, krylov=args.krylov, dp_apex=args.dp_apex)
-        run_steps(st, cfgt, 0, args.warmup + args.steps)
-        h = st.residual_history
-        out['mixed_tol_1e-11_vs_fp64_field_rel_err'] = dict(
-            field_errors(st, solvers['fp64']), newton_iters_last_step=st.newton_iterations,
-            relative_residual=h[-1] / h[0] if h and h[0] > 0 else None)
-        del st
-    if args.c2_steps > 0:
-        out['c2_j2_bar'] = c2_bench(args.c2_steps)
-    if args.c3_steps > 0:
-        out['c3_consolidation'] = c3_bench(args.c3_steps)
-    if args.adjoint_steps > 0:
-        out['adjoint'] = adjoint_bench(args.adjoint_steps, krylov=args.krylov)
+    # warm-up load steps 0 .. W-1 in the mixed form; both forms start the timed steps from its state
+    smx, cfg = build_solver(cells, 'mixed', args)
+    warm = run_steps(smx, cfg, 0, args.warmup, log=log and 'warmup')
+    out = {'metric': METRIC, 'unit': 'Newton steps/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+           'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+           'dtype': 'f32+f64 (FP32 Jacobian/Krylov/FP16 smoother copies, FP64 residual and refinement)',
+           'data': 'synthetic'}
+    shape = ('shape factor %g' % args.shape_factor) if args.material == 'Drucker-Prager' else ''
+    out['config'] = {'workload': '3D u-p indentation %dx%dx%d cells, %d particles, %s%s, indenter 0.2 x 0.2 m pushed '
+                                 '0.05 m in 40 load steps (start: %s), load steps %d-%d timed to 1e-8 relative '
+                                 'residual' % (cells + (smx.n_particles, args.material,
+                                                        (' phi 30 c 5 psi 0 apex %s, %s' % (args.dp_apex, shape))
+                                                        if args.material == 'Drucker-Prager' else '',
+                                                        args.footing_start, args.warmup,
+                                                        args.warmup + args.steps - 1)),
+                     'nodes': smx.n_nodes, 'dofs': smx.n_matrix_size, 'substep_levels': args.substeps}
+    out['warmup_steps'] = summarize(warm, args.warmup)
+    if not out['warmup_steps']['all_converged']:
+        out['value'] = None
+        out['error'] = 'a warm-up load step did not converge (see warmup_steps.failed_step)'
+        print(json.dumps(out))
+        return
+    snap = smx.state_dict()
+
+    # timed: mixed form (the value), keeping end-of-step fields for the comparison
+    from geowarp_amd.solvers._engine import PhaseTimer
+    mixed_fields = {}
+    smx.engine.timers = PhaseTimer()
+    recs = []
+    smx.engine.stats['krylov_iters'].clear()
+    for k in range(args.warmup, args.warmup + args.steps):
+        r = run_steps(smx, cfg, k, 1, log=log and 'mixed')
+        recs += r
+        mixed_fields[k] = fields(smx)
+        if not r[-1]['converged']:
+            break
+    phases = smx.engine.timers.summary()
+    smx.engine.timers = None
+    res_mx = summarize(recs, args.steps)
+    res_mx['phases_ms'] = {k: round(v, 2) for k, v in phases.items()}
+    res_mx['per_step'] = recs
+    out['mixed'] = res_mx
+    out['config']['nnzb'] = smx.engine.nnzb
+    out['value'] = res_mx['newton_per_s'] if res_mx['all_converged'] else None
+    out['ms_per_step'] = res_mx['ms_per_load_step']
+    if not res_mx['all_converged']:
+        out['error'] = 'mixed form: load step %s did not converge' % res_mx.get('failed_step')
+
+    # timed: FP64 form from the same state, as many of the same steps as the budget allows
+    if not args.no_fp64:
+        s64, _ = build_solver(cells, 'fp64', args)
+        s64.load_state_dict(snap)
+        s64.engine.timers = PhaseTimer()
+        recs = []
+        for k in range(args.warmup, args.warmup + args.steps):
+            elapsed = time.perf_counter() - t_start
+            if recs:
+                per = sum(x['seconds'] for x in recs) / len(recs)
+                if elapsed + 1.5 * per > 0.8 * args.budget:
+                    break
+            r = run_steps(s64, cfg, k, 1, log=log and 'fp64')
+            recs += r
+            if not r[-1]['converged']:
+                break
+        phases = s64.engine.timers.summary()
+        s64.engine.timers = None
+        res64 = summarize(recs, len(recs))
+        res64['phases_ms'] = {k: round(v, 2) for k, v in phases.items()}
+        res64['per_step'] = recs
+        out['fp64'] = res64
+        n = len(recs)
+        if n and res64['all_converged']:
+            mx_sec = sum(x['seconds'] for x in res_mx['per_step'][:n])
+            out['mixed_vs_fp64_speedup'] = round(res64['seconds'] / mx_sec, 3) if mx_sec > 0 else None
+            out['mixed_vs_fp64_steps_compared'] = [args.warmup, args.warmup + n - 1]
+            last = args.warmup + n - 1
+            if last in mixed_fields:
+                out['mixed_vs_fp64_field_rel_err'] = field_errors(mixed_fields[last], fields(s64))
+        del s64
+        torch.cuda.empty_cache()
+
+    # roofline of the dominant kernel of the timed form and of the Krylov SpMV (last Jacobian)
+    smo = smoother_roofline(smx)
+    roof = spmv_roofline(smx)
+    out['roofline'] = {'bound': 'hbm', 'achieved': round(smo['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                       'frac': round(smo['achieved_GBs'] / HBM_PEAK_GBS, 4),
+                       'traffic': measured_traffic(cells, args.material, 'k_level'),
+                       'kernel': smo['kernel'], 'avg_us': round(smo['avg_us'], 2),
+                       'algorithmic_bytes': smo['bytes_per_launch']}
+    out['roofline_krylov_spmv'] = {'bound': 'hbm', 'achieved': round(roof['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS,
+                                   'unit': 'GB/s', 'frac': round(roof['achieved_GBs'] / HBM_PEAK_GBS, 4),
+                                   'traffic': measured_traffic(cells, args.material),
+                                   'kernel': roof['kernel'], 'avg_us': round(roof['avg_us'], 2),
+                                   'algorithmic_bytes': roof['bytes_per_launch']}
     if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline()
+        out['cpu_baseline'] = cpu_baseline(args)
+    if args.extras:
+        if args.c2_steps > 0 and time.perf_counter() - t_start < args.budget:
+            out['c2_j2_bar'] = c2_bench(args.c2_steps)
+        if args.c3_steps > 0 and time.perf_counter() - t_start < args.budget:
+            out['c3_consolidation'] = c3_bench(args.c3_steps)
+        if args.adjoint_steps > 0 and time.perf_counter() - t_start < args.budget:
+            out['adjoint'] = adjoint_bench(args.adjoint_steps, krylov=args.krylov)
+    out['wall_seconds'] = round(time.perf_counter() - t_start, 1)
     print(json.dumps(out))
 
 
