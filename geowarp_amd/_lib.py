@@ -89,6 +89,16 @@ _SIGS = {
     'gm_node_jacobian': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern), _P,
                                  c_int, _P, _P, c_int, c_int, _P]),
     'gm_node_jacobian_work_bytes': (ctypes.c_size_t, [ctypes.POINTER(gm_grid), c_int]),
+    'gm_jacobian_bins': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), _P, c_int, _P, ctypes.c_size_t,
+                                 _P]),
+    'gm_jacobian_gather': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern),
+                                   c_int, _P, _P, _P, c_int, _P]),
+    'gm_mass_work_bytes': (ctypes.c_size_t, [ctypes.POINTER(gm_grid)]),
+    'gm_residual_work_bytes': (ctypes.c_size_t, [ctypes.POINTER(gm_grid)]),
+    'gm_node_residual_binned': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), _P, _P, _P, _P,
+                                        ctypes.c_size_t, _P]),
+    'gm_mass_binned': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern),
+                               c_double, _P, _P, ctypes.c_size_t, _P]),
     'gm_node_jacobian_binned': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins),
                                         ctypes.POINTER(gm_pattern), _P, c_int, _P, _P, c_int, _P, ctypes.c_size_t,
                                         _P]),

@@ -67,11 +67,13 @@ struct Mfma<double> {
 };
 
 // z index of E feeding column dof c through shape-function component beta
-// (0: w, 1-3: dw/dx_a, 4: w_avg), or -1 where that product is zero.
+// (0: w, 1-3: dw/dx_a, 4: w_avg), or ZERO (a column of E that holds zeros) where
+// that product vanishes.
+constexpr int ZERO = NZ;
 __device__ __forceinline__ int zsel(int c, int beta) {
   if (c < 3) {
-    if (beta == 0) return c == 2 ? Z_DU : -1;
-    if (beta == 4) return -1;
+    if (beta == 0) return c == 2 ? Z_DU : ZERO;
+    if (beta == 4) return ZERO;
     return (beta - 1) * 3 + c;
   }
   if (beta == 0) return Z_P;
@@ -103,21 +105,25 @@ __global__ void __launch_bounds__(256, 2) k_jac_bin(gm_grid g, const double* x_s
   constexpr int PG = sizeof(TC) == 4 ? 8 : 4;
   constexpr int KST = 5 * PG / 4;  // K steps of 4 per group
   __shared__ TC AXT[PG][3][9];
-  __shared__ TC E[PG][27][4][NZ];
+  // E[p][I][r][z] plus a zero node row (I = 27) and a zero column (z = ZERO): every
+  // A operand is then an unconditional LDS read
+  __shared__ TC E[PG][28][4][NZ + 1];
   __shared__ TC BT[PG][5][32];  // B operand g_p[J][beta], J padded to 32 with zeros
   const int b = blockIdx.x;
   const int ks = cell_start[b], ke = cell_start[b + 1];
   if (ks == ke) return;  // uniform across the workgroup
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int q = t; q < PG * 4 * (NZ + 1); q += 256) E[q / (4 * (NZ + 1))][27][(q / (NZ + 1)) % 4][q % (NZ + 1)] = 0;
+  for (int q = t; q < PG * 27 * 4; q += 256) E[q / 108][(q / 4) % 27][q % 4][ZERO] = 0;
   typedef typename Mfma<TC>::acc ACC;
   ACC acc[4][4];
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[s][c] = ACC{0, 0, 0, 0};
-  // output tiles tt = wave + 4 s (tt < 14): row tile mt = tt / 2 (16 rows (I, r)),
-  // column tile nt = tt % 2 (16 columns J)
-  const int ntile = (wave < 2) ? 4 : 3;
+  // output tiles tt = wave + 4 s: row tile mt = tt / 2 (16 rows (I, r)), column tile
+  // nt = tt % 2 (16 columns J); tiles 14 and 15 (the fourth of waves 2 and 3) have
+  // I >= 28, read the zero row and are not stored
   constexpr int NQ = (PG * 4 * NZ + 255) / 256;  // (p, r, z) items per thread
   for (int p0 = ks; p0 < ke; p0 += PG) {
     const int np = min(PG, ke - p0);
@@ -218,20 +224,21 @@ __global__ void __launch_bounds__(256, 2) k_jac_bin(gm_grid g, const double* x_s
       int zc[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) zc[c] = zsel(c, beta);
+      TC av[4][4];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        if (s < ntile) {
-          const int tt = wave + 4 * s, mt = tt >> 1, nt = tt & 1;
-          const int rt = lane & 15;
-          const int I = mt * 4 + (Mfma<TC>::r_major ? (rt & 3) : (rt >> 2));
-          const int r = Mfma<TC>::r_major ? (rt >> 2) : (rt & 3);
-          const TC bk = nt ? b1 : b0;
+        const int tt = wave + 4 * s, mt = tt >> 1;
+        const int rt = lane & 15;
+        const int I = min(mt * 4 + (Mfma<TC>::r_major ? (rt & 3) : (rt >> 2)), 27);
+        const int r = Mfma<TC>::r_major ? (rt >> 2) : (rt & 3);
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const TC av = (I < 27 && zc[c] >= 0) ? E[p][I][r][zc[c]] : (TC)0;
-            acc[s][c] = Mfma<TC>::step(av, bk, acc[s][c]);
-          }
-        }
+        for (int c = 0; c < 4; ++c) av[s][c] = E[p][I][r][zc[c]];
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const TC bk = ((wave + 4 * s) & 1) ? b1 : b0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[s][c] = Mfma<TC>::step(av[s][c], bk, acc[s][c]);
       }
     }
   }
@@ -240,7 +247,6 @@ __global__ void __launch_bounds__(256, 2) k_jac_bin(gm_grid g, const double* x_s
   TC* out = cb + (long)b * CB_BIN;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    if (s >= ntile) break;
     const int tt = wave + 4 * s, mt = tt >> 1, nt = tt & 1;
     const int I = mt * 4 + (lane >> 4), J = nt * 16 + (lane & 15);
     if (I < 27 && J < 27) {
@@ -314,23 +320,234 @@ __global__ void __launch_bounds__(512) k_jac_gather(gm_grid g, const int* cell_s
   }
 }
 
+// ================================================================== residual (3D u-p)
+// r_I = sum_p L_I Y_p per bin (node I_loc of the bin's 3x3x3 box, its particles in
+// bin order), then per node the fixed-order sum over the <= 27 bins holding it.
+__global__ void __launch_bounds__(64) k_res_bin(gm_grid g, const double* x_s, const double* lp_s,
+                                                const int* cell_start, const double* Y, double* cb) {
+  constexpr int PB = 16;
+  __shared__ double AX[PB][3][9];
+  const int b = blockIdx.x;
+  const int ks = cell_start[b], ke = cell_start[b + 1];
+  if (ks == ke) return;  // uniform across the workgroup
+  const int t = threadIdx.x;
+  const int ix = t % 3, iy = (t / 3) % 3, iz = t / 9;
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int p0 = ks; p0 < ke; p0 += PB) {
+    const int np = min(PB, ke - p0);
+    __syncthreads();
+    if (t < PB * 3) {
+      const int p = t / 3, a = t % 3;
+      if (p < np) {
+        const Axis ax = gimp_axis(x_s[(long)(p0 + p) * 3 + a], lp_s[(long)(p0 + p) * 3 + a], g.h[a], g.inv_h[a]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          AX[p][a][i] = ax.w[i];
+          AX[p][a][3 + i] = ax.g[i];
+          AX[p][a][6 + i] = ax.wa[i];
+        }
+      }
+    }
+    __syncthreads();
+    if (t < 27) {
+      for (int p = 0; p < np; ++p) {
+        const double wx = AX[p][0][ix], wy = AX[p][1][iy], wz = AX[p][2][iz];
+        const double w = wx * wy * wz;
+        const double G[3] = {AX[p][0][3 + ix] * wy * wz, wx * AX[p][1][3 + iy] * wz, wx * wy * AX[p][2][3 + iz]};
+        const double wa = AX[p][0][6 + ix] * AX[p][1][6 + iy] * AX[p][2][6 + iz];
+        const double* yp = Y + (long)(p0 + p) * NY;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          double sv = 0.0;
+#pragma unroll
+          for (int bb = 0; bb < 3; ++bb) sv += G[bb] * yp[bb * 3 + c];
+          r[c] += sv + w * yp[Y_T + c];
+        }
+        double sv = w * yp[Y_Q];
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) sv += G[bb] * yp[Y_M + bb];
+        sv += (w - wa) * yp[Y_S];
+        r[3] += sv;
+      }
+    }
+  }
+  if (t < 27)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) cb[((long)b * 27 + t) * 4 + c] = r[c];
+}
+
+__global__ void __launch_bounds__(256) k_res_gather(gm_grid g, const int* cell_start, const double* cb,
+                                                    const uint8_t* freem, double* rhs) {
+  const int I = blockIdx.x * 64 + threadIdx.x / 4, c = threadIdx.x % 4;
+  if (I >= g.n_nodes) return;
+  const int nx1 = g.nx + 1, nxy = (g.nx + 1) * (g.ny + 1);
+  const int ci[3] = {I % nx1, (I / nx1) % (g.ny + 1), I / nxy};
+  double s = 0.0;
+  for (int o = 0; o < 27; ++o) {
+    const int ox = o % 3, oy = (o / 3) % 3, oz = o / 9;
+    const int bc[3] = {ci[0] - ox, ci[1] - oy, ci[2] - oz};
+    if (bc[0] < 0 || bc[0] > g.nx || bc[1] < 0 || bc[1] > g.ny || bc[2] < 0 || bc[2] > g.nz) continue;
+    const int bn = bc[0] + bc[1] * nx1 + bc[2] * nxy;
+    if (cell_start[bn + 1] == cell_start[bn]) continue;
+    s += cb[((long)bn * 27 + o) * 4 + c];
+  }
+  rhs[I * 4 + c] = freem[I * 4 + c] ? s : 0.0;
+}
+
+extern "C" size_t gm_residual_work_bytes(const gm_grid* g) {
+  return g ? (size_t)g->n_nodes * 27 * 4 * sizeof(double) : 0;
+}
+
+extern "C" int gm_node_residual_binned(const gm_grid* g, const gm_bins* b, const double* Y, const uint8_t* freem,
+                                       double* rhs, void* work, size_t work_bytes, hipStream_t s) {
+  GM_REQUIRE(g && b && Y && freem && rhs && work, "gm_node_residual_binned: null argument");
+  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_node_residual_binned: the 3D u-p form only");
+  GM_REQUIRE(work_bytes >= gm_residual_work_bytes(g),
+             "gm_node_residual_binned: workspace smaller than gm_residual_work_bytes");
+  const int n = g->n_nodes;
+  if (n == 0) return 0;
+  k_res_bin<<<n, 64, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, Y, (double*)work);
+  GM_LAUNCH_CHECK();
+  k_res_gather<<<gm_blocks(n, 64), 256, 0, s>>>(*g, b->cell_start, (const double*)work, freem, rhs);
+  GM_LAUNCH_CHECK();
+  return 0;
+}
+
+// ================================================================== consistent mass (3D)
+// M_IJ = sum_p m_p w_I w_J in the same node pattern (one value per block slot),
+// assembled like the Jacobian: per bin the 27 x 27 products of its particles (in
+// bin order), then per row the fixed-order sum over the bins holding both nodes.
+__global__ void __launch_bounds__(256) k_mass_bin(gm_grid g, const double* x_s, const double* lp_s,
+                                                  const int* cell_start, double p_mass, double* cb) {
+  constexpr int PB = 32;
+  __shared__ double W[PB][3][3];
+  const int b = blockIdx.x;
+  const int ks = cell_start[b], ke = cell_start[b + 1];
+  if (ks == ke) return;  // uniform across the workgroup
+  const int t = threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int p0 = ks; p0 < ke; p0 += PB) {
+    const int np = min(PB, ke - p0);
+    __syncthreads();
+    if (t < PB * 3) {
+      const int p = t / 3, a = t % 3;
+      if (p < np) {
+        const Axis ax = gimp_axis(x_s[(long)(p0 + p) * 3 + a], lp_s[(long)(p0 + p) * 3 + a], g.h[a], g.inv_h[a]);
+        W[p][a][0] = ax.w[0];
+        W[p][a][1] = ax.w[1];
+        W[p][a][2] = ax.w[2];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int q = t + 256 * j;
+      if (q < 729) {
+        const int I = q / 27, J = q % 27;
+        const int ix = I % 3, iy = (I / 3) % 3, iz = I / 9, jx = J % 3, jy = (J / 3) % 3, jz = J / 9;
+        for (int p = 0; p < np; ++p) {
+          const double wI = W[p][0][ix] * W[p][1][iy] * W[p][2][iz];
+          const double wJ = W[p][0][jx] * W[p][1][jy] * W[p][2][jz];
+          acc[j] += wI * wJ * p_mass;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int q = t + 256 * j;
+    if (q < 729) cb[(long)b * 729 + q] = acc[j];
+  }
+}
+
+__global__ void __launch_bounds__(128) k_mass_gather(gm_grid g, const int* cell_start, const double* cb,
+                                                     const int* row_ptr, const uint64_t* row_mask, double* mvals) {
+  __shared__ int SLOT_BIT[125];
+  __shared__ int BIN[27];
+  const int I = blockIdx.x, t = threadIdx.x;
+  const uint64_t m0 = row_mask[2 * I], m1 = row_mask[2 * I + 1];
+  const int nx1 = g.nx + 1, nxy = (g.nx + 1) * (g.ny + 1);
+  const int ci[3] = {I % nx1, (I / nx1) % (g.ny + 1), I / nxy};
+  if (t < 125) {
+    const uint64_t w = (t < 64) ? m0 : m1;
+    const int sh = t & 63;
+    if ((w >> sh) & 1ull) {
+      const int below = (t < 64) ? __popcll(m0 & ((1ull << sh) - 1ull))
+                                 : __popcll(m0) + __popcll(m1 & ((1ull << sh) - 1ull));
+      SLOT_BIT[below] = t;
+    }
+  }
+  if (t < 27) {
+    const int ox = t % 3, oy = (t / 3) % 3, oz = t / 9;
+    const int bc[3] = {ci[0] - ox, ci[1] - oy, ci[2] - oz};
+    int bn = -1;
+    if (bc[0] >= 0 && bc[0] <= g.nx && bc[1] >= 0 && bc[1] <= g.ny && bc[2] >= 0 && bc[2] <= g.nz) {
+      bn = bc[0] + bc[1] * nx1 + bc[2] * nxy;
+      if (cell_start[bn + 1] == cell_start[bn]) bn = -1;
+    }
+    BIN[t] = bn;
+  }
+  __syncthreads();
+  const int nnz = __popcll(m0) + __popcll(m1);
+  if (t >= nnz) return;
+  const int bit = SLOT_BIT[t];
+  const int d[3] = {bit % 5 - 2, (bit / 5) % 5 - 2, bit / 25 - 2};
+  double s = 0.0;
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    const int kx = d[0] + o % 3, ky = d[1] + (o / 3) % 3, kz = d[2] + o / 9;
+    const int bn = BIN[o];
+    if (bn >= 0 && kx >= 0 && kx < 3 && ky >= 0 && ky < 3 && kz >= 0 && kz < 3)
+      s += cb[(long)bn * 729 + o * 27 + (kx + 3 * ky + 9 * kz)];
+  }
+  mvals[row_ptr[I] + t] = s;
+}
+
+extern "C" size_t gm_mass_work_bytes(const gm_grid* g) { return g ? (size_t)g->n_nodes * 729 * sizeof(double) : 0; }
+
+extern "C" int gm_mass_binned(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, double p_mass,
+                              double* mvals, void* work, size_t work_bytes, hipStream_t s) {
+  GM_REQUIRE(g && b && pat && mvals && work, "gm_mass_binned: null argument");
+  GM_REQUIRE(g->dim == 3, "gm_mass_binned: 3D grids only");
+  GM_REQUIRE(work_bytes >= gm_mass_work_bytes(g), "gm_mass_binned: workspace smaller than gm_mass_work_bytes");
+  const int n = g->n_nodes;
+  if (n == 0) return 0;
+  k_mass_bin<<<n, 256, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, p_mass, (double*)work);
+  GM_LAUNCH_CHECK();
+  k_mass_gather<<<n, 128, 0, s>>>(*g, b->cell_start, (const double*)work, pat->row_ptr, pat->row_mask, mvals);
+  GM_LAUNCH_CHECK();
+  return 0;
+}
+
 extern "C" size_t gm_node_jacobian_work_bytes(const gm_grid* g, int tangent_fp32) {
   if (!g) return 0;
   return (size_t)g->n_nodes * (size_t)CB_BIN * (tangent_fp32 ? sizeof(float) : sizeof(double));
 }
 
-extern "C" int gm_node_jacobian_binned(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, const void* D,
-                                       int tangent_fp32, const uint8_t* freem, void* vals, int vals_fp32,
-                                       void* work, size_t work_bytes, hipStream_t s) {
-  GM_REQUIRE(g && b && pat && D && freem && vals && work, "gm_node_jacobian_binned: null argument");
-  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_node_jacobian_binned: the 3D u-p form only");
+extern "C" int gm_jacobian_bins(const gm_grid* g, const gm_bins* b, const void* D, int tangent_fp32, void* work,
+                                size_t work_bytes, hipStream_t s) {
+  GM_REQUIRE(g && b && D && work, "gm_jacobian_bins: null argument");
+  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_jacobian_bins: the 3D u-p form only");
   GM_REQUIRE(work_bytes >= gm_node_jacobian_work_bytes(g, tangent_fp32),
-             "gm_node_jacobian_binned: workspace smaller than gm_node_jacobian_work_bytes");
+             "gm_jacobian_bins: workspace smaller than gm_node_jacobian_work_bytes");
+  const int n = g->n_nodes;
+  if (n == 0) return 0;
+  if (tangent_fp32)
+    k_jac_bin<float, float><<<n, 256, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, (const float*)D, (float*)work);
+  else
+    k_jac_bin<double, double><<<n, 256, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, (const double*)D,
+                                                (double*)work);
+  GM_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int gm_jacobian_gather(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, int tangent_fp32,
+                                  const void* work, const uint8_t* freem, void* vals, int vals_fp32, hipStream_t s) {
+  GM_REQUIRE(g && b && pat && work && freem && vals, "gm_jacobian_gather: null argument");
+  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_jacobian_gather: the 3D u-p form only");
   const int n = g->n_nodes;
   if (n == 0) return 0;
   if (tangent_fp32) {
-    k_jac_bin<float, float><<<n, 256, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, (const float*)D, (float*)work);
-    GM_LAUNCH_CHECK();
     if (vals_fp32)
       k_jac_gather<float, float><<<n, 512, 0, s>>>(*g, b->cell_start, (const float*)work, freem, pat->row_ptr,
                                                    pat->row_mask, (float*)vals);
@@ -338,9 +555,6 @@ extern "C" int gm_node_jacobian_binned(const gm_grid* g, const gm_bins* b, const
       k_jac_gather<float, double><<<n, 512, 0, s>>>(*g, b->cell_start, (const float*)work, freem, pat->row_ptr,
                                                     pat->row_mask, (double*)vals);
   } else {
-    k_jac_bin<double, double><<<n, 256, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, (const double*)D,
-                                                (double*)work);
-    GM_LAUNCH_CHECK();
     if (vals_fp32)
       k_jac_gather<double, float><<<n, 512, 0, s>>>(*g, b->cell_start, (const double*)work, freem, pat->row_ptr,
                                                     pat->row_mask, (float*)vals);
@@ -350,4 +564,13 @@ extern "C" int gm_node_jacobian_binned(const gm_grid* g, const gm_bins* b, const
   }
   GM_LAUNCH_CHECK();
   return 0;
+}
+
+extern "C" int gm_node_jacobian_binned(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, const void* D,
+                                       int tangent_fp32, const uint8_t* freem, void* vals, int vals_fp32,
+                                       void* work, size_t work_bytes, hipStream_t s) {
+  GM_REQUIRE(g && b && pat && D && freem && vals && work, "gm_node_jacobian_binned: null argument");
+  int rc = gm_jacobian_bins(g, b, D, tangent_fp32, work, work_bytes, s);
+  if (rc) return rc;
+  return gm_jacobian_gather(g, b, pat, tangent_fp32, work, freem, vals, vals_fp32, s);
 }

@@ -89,8 +89,9 @@ __global__ void __launch_bounds__(T) k_spmv(int n_rows, const int* row_ptr, cons
 static int g_spmv_threads[5][2] = {{256, 256}, {256, 256}, {256, 256}, {256, 256}, {256, 256}};
 
 // Lanes per block row: 3D u-p rows hold 27-125 blocks (16 lanes measured 4-6 % faster than 8
-// on the 64x64x32-cell indentation Jacobian in both precisions); 2D rows hold at most 25.
-static constexpr int spmv_lanes(int nb) { return nb >= 4 ? 16 : (nb == 3 ? 8 : 4); }
+// on the 64x64x32-cell indentation Jacobian in both precisions); 2D rows hold at most 25
+// blocks; scalar rows (nb = 1) are the consistent-mass matrices, 125 entries in 3D.
+static constexpr int spmv_lanes(int nb) { return nb >= 4 ? 16 : (nb == 3 ? 8 : (nb == 2 ? 4 : 16)); }
 
 template <int NB, typename TM, typename TX, typename TY>
 static void launch_spmv(int n_rows, const int* rp, const int* ci, const TM* v, const TX* x, TY* y, const TY* w1,

@@ -1246,7 +1246,14 @@ __global__ void k_p2g(gm_grid g, int nb, const double* x_s, const double* lp_s, 
     double w, G[3], wa;
     node_geom<DIM>(ax, o, w, G, wa);
     if (threshold < 0.0 || w > threshold) a_on = true;
-    if (mvals || mrhs) {
+    if (DIM == 3 && !mvals && mrhs) {
+      // sum of the particle's weights over its support, one axis at a time (the
+      // 3D mass matrix comes from gm_mass_binned)
+      double wsum = 1.0;
+#pragma unroll
+      for (int aa = 0; aa < 3; ++aa) wsum *= ax[aa].w[0] + ax[aa].w[1] + ax[aa].w[2];
+      rhs += w * wsum * p_mass * p_part[p];
+    } else if (mvals || mrhs) {
       double wsum = 0.0;
       uint8_t ex = ext_s[k];
       int e[3];

@@ -181,6 +181,7 @@ class _QuasiStaticBase:
         n_grid = (self.n_grid_x, self.n_grid_y) if self.dim == 2 else (self.n_grid_x, self.n_grid_y, self.n_grid_z)
         self.engine = Engine(self.dim, self.coupled, n_grid, spacing, device=self.device, precision=self.precision)
         self.engine.mixed_assembly = it.get('mixed_assembly', 'fp32')
+        self.engine.mg_lag = bool(it.get('mg_lag', False))
         self.engine.mg_options = mg_opts
         dev = self.device
         self.dofStruct = DofStruct(self.n_matrix_size, dev)
@@ -282,14 +283,19 @@ class _QuasiStaticBase:
         tm = eng.timers
         eta = None
         nr_prev = None
+        self._ls_ready = None  # norm of an accepted line-search trial whose stage had the tangent
         for it in range(self.n_iter):
             if tm is not None:
                 tm.mark('start')
-            eng.stage(args, want_tangent=True)
-            if tm is not None:
-                tm.mark('particle_stage')
-            eng.residual(x, lp)
-            eng.norm2(eng.rhs, 0)
+            reuse = self._ls_ready
+            self._ls_ready = None
+            if reuse is None:
+                eng.stage(args, want_tangent=True)
+                if tm is not None:
+                    tm.mark('particle_stage')
+                eng.residual(x, lp)
+                eng.norm2(eng.rhs, 0)
+            # else: the accepted line-search trial left this iterate's tangent, residual and norm
             if after_residual is not None:
                 after_residual()
             if tm is not None:
@@ -299,7 +305,7 @@ class _QuasiStaticBase:
                 eng.autotune_spmv()
             if tm is not None:
                 tm.mark('node_jacobian')
-            nr = math.sqrt(max(eng.scal[0].item(), 0.0))
+            nr = reuse if reuse is not None else math.sqrt(max(eng.scal[0].item(), 0.0))
             tol_lin = self.linear_tol
             if self.forcing == 'ew':
                 if nr_prev is None:
@@ -369,26 +375,33 @@ class _QuasiStaticBase:
     def _line_search(self, args, u_new, nr, after_residual, max_halvings=6):
         """Backtracking on the Newton correction (not in the reference, which
         always takes the full step): accept u + a*delta for the first
-        a = 1, 1/2, ... whose residual norm is finite and below (1 - 1e-4 a) nr."""
+        a = 1, 1/2, ... whose residual norm is finite and below (1 - 1e-4 a) nr.
+        The full step is evaluated with the particle tangent: when it is accepted,
+        the next Newton iteration starts from its tangent, residual and norm
+        (self._ls_ready) instead of evaluating them again."""
         eng = self.engine
         base = u_new.clone()
         a = 1.0
         finite = False
+        accepted = False
         for _ in range(max_halvings + 1):
             u_new.copy_(base)
             eng.axpy(a, eng.delta, u_new)
-            eng.stage(args, want_tangent=False)
+            eng.stage(args, want_tangent=(a == 1.0))
             eng.residual(self.x_particles, self.GIMP_lp)
             eng.norm2(eng.rhs, 0)
             trial = math.sqrt(max(eng.scal[0].item(), 0.0))
             finite = math.isfinite(trial)
             if finite and trial <= (1.0 - 1e-4 * a) * nr:
+                accepted = True
                 break
             if _ < max_halvings:
                 a *= 0.5
         if not finite:
             u_new.copy_(base)
             raise L.GMError('line search: no finite residual along the Newton correction (down to step %g)' % a)
+        if accepted and a == 1.0:
+            self._ls_ready = trial
         self.line_search_steps.append(a)
 
     def _ew_forcing(self, nr, nr_prev, eta_prev, r0):

@@ -102,10 +102,16 @@ class Engine:
         self.penalty = None
         self.penalty_pc = True
         self.vals_pc = None
-        # node Jacobian of the 3D u-p form: 'bins' (per-bin products on the matrix cores into a
-        # workspace of n_nodes x 11664 values, then a fixed-order gather) or 'rows' (one
-        # workgroup per block row, no workspace)
+        # node Jacobian and residual of the 3D u-p form: 'bins' (per-bin sums -- the Jacobian's
+        # as products on the matrix cores -- into a workspace of up to n_nodes x 11664 values,
+        # then a fixed-order gather) or 'rows' (one workgroup or thread per node row, no
+        # workspace)
         self.jacobian_kernel = os.environ.get('GM_JACOBIAN_KERNEL', 'bins')
+        # multigrid: rebuild the coarse Galerkin operators at the first solve of a load step
+        # only (later Newton corrections refresh level 0 and keep them), unless a solve then
+        # needs three times the iterations of the step's first
+        self.mg_lag = False
+        self.step_id = 0
 
     # ------------------------------------------------------------------ buffers
     def _alloc_vals(self):
@@ -176,6 +182,7 @@ class Engine:
     def begin_step(self, x, lp, pattern=True):
         """Bin particles by stencil base node and build the block-CSR pattern."""
         lib = L.lib()
+        self.step_id += 1
         self.ensure_particles(x.shape[0])
         self.bins = self._bins()
         L.check(lib.gm_bin_particles(L.ctypes.byref(self.grid), L.ptr(x), L.ptr(lp), L.ctypes.byref(self.bins),
@@ -216,10 +223,18 @@ class Engine:
             if self.mvals is None or self.mvals.numel() < self.cap:
                 self.mvals = torch.zeros(self.cap, dtype=torch.float64, device=self.dev)
             mrhs = torch.zeros(self.n_nodes, dtype=torch.float64, device=self.dev)
+        binned_mass = mass and self.dim == 3
         L.check(lib.gm_p2g(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ctypes.byref(self.pattern),
                            float(threshold), L.ptr(pbc_flag), float(node_h_last),
                            float(p_mass), L.ptr(p_part), L.ptr(self.act), L.ptr(self.bflag),
-                           L.ptr(self.mvals) if mass else None, L.ptr(mrhs), _stream()), 'gm_p2g')
+                           L.ptr(self.mvals) if mass and not binned_mass else None, L.ptr(mrhs), _stream()),
+                'gm_p2g')
+        if binned_mass:
+            g = L.ctypes.byref(self.grid)
+            need = int(lib.gm_mass_work_bytes(g))
+            w = self._bins_workspace(need)
+            L.check(lib.gm_mass_binned(g, L.ctypes.byref(self.bins), L.ctypes.byref(self.pattern), float(p_mass),
+                                       L.ptr(self.mvals), L.ptr(w), need, _stream()), 'gm_mass_binned')
         L.check(lib.gm_free_mask(self.n_nodes * self.nb, L.ptr(self.bflag), L.ptr(self.act), L.ptr(self.freem),
                                  _stream()), 'gm_free_mask')
         return mrhs
@@ -236,7 +251,7 @@ class Engine:
         it = L.c_int(0)
         rr = L.c_double(0.0)
         L.check(lib.gm_bicgstab(1, 0, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx), L.ptr(self.mvals),
-                                L.ptr(mrhs), L.ptr(xp), L.ctypes.byref(ws), tol, maxiter, 16, L.ctypes.byref(it),
+                                L.ptr(mrhs), L.ptr(xp), L.ctypes.byref(ws), tol, maxiter, 4, L.ctypes.byref(it),
                                 L.ctypes.byref(rr), _stream()), 'gm_bicgstab(mass)')
         self.mass_iters = it.value
         self.mass_relres = rr.value
@@ -260,7 +275,16 @@ class Engine:
                 'gm_particle_stage')
 
     def residual(self, x, lp):
-        L.check(L.lib().gm_node_residual(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ptr(self.Y),
+        g = L.ctypes.byref(self.grid)
+        if self.dim == 3 and self.coupled and self.jacobian_kernel == 'bins':
+            # per-bin sums into the shared bin workspace, then a fixed-order gather per node
+            need = int(L.lib().gm_residual_work_bytes(g))
+            w = self._bins_workspace(need)
+            L.check(L.lib().gm_node_residual_binned(g, L.ctypes.byref(self.bins), L.ptr(self.Y), L.ptr(self.freem),
+                                                    L.ptr(self.rhs), L.ptr(w), need, _stream()),
+                    'gm_node_residual_binned')
+            return
+        L.check(L.lib().gm_node_residual(g, L.ctypes.byref(self.bins), L.ptr(self.Y),
                                          L.ptr(self.freem), L.ptr(self.rhs), int(self.coupled), _stream()),
                 'gm_node_residual')
 
@@ -271,7 +295,9 @@ class Engine:
     def multigrid(self, fp32):
         key = ('mg', bool(fp32))
         if key not in self._ws:
-            self._ws[key] = [Multigrid(self, fp32, **self.mg_options), -1]
+            # [hierarchy, Jacobian version it was set up for, load step of its last full
+            # setup, GMRES iterations of that step's first solve, rebuild at the next setup]
+            self._ws[key] = [Multigrid(self, fp32, **self.mg_options), -1, -1, 0, False]
         return self._ws[key]
 
     def autotune_spmv(self, reps=10):
@@ -304,14 +330,20 @@ class Engine:
             return
         self._assemble(fp32, self.vals, fp32)
 
+    def _bins_workspace(self, nbytes):
+        """Per-bin workspace of the binned 3D assemblies (Jacobian, consistent mass),
+        grown on demand; the assemblies run in stream order, so they share it."""
+        w = self._ws.get('bins')
+        if w is None or w.numel() < nbytes:
+            self._ws.pop('bins', None)
+            w = self._ws['bins'] = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+        return w
+
     def _assemble(self, tangent_fp32, vals, vals_fp32):
         g, b, pat = L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ctypes.byref(self.pattern)
         if self.dim == 3 and self.coupled and self.jacobian_kernel == 'bins':
             need = int(L.lib().gm_node_jacobian_work_bytes(g, int(tangent_fp32)))
-            w = self._ws.get('jacobian_bins')
-            if w is None or w.numel() < need:
-                self._ws.pop('jacobian_bins', None)
-                w = self._ws['jacobian_bins'] = torch.empty(need, dtype=torch.uint8, device=self.dev)
+            w = self._bins_workspace(need)
             L.check(L.lib().gm_node_jacobian_binned(g, b, pat, L.ptr(self.D), int(tangent_fp32), L.ptr(self.freem),
                                                     L.ptr(vals), int(vals_fp32), L.ptr(w), need, _stream()),
                     'gm_node_jacobian_binned')
@@ -394,12 +426,21 @@ class Engine:
         if method == 'mg':
             entry = self.multigrid(fp32)
             mgh = entry[0]
+            full = True
             if entry[1] != self.jac_version:
+                full = not self.mg_lag or entry[2] != self.step_id or entry[4] or mgh.ublock
                 hv, op = self._hierarchy_matrix() if not mgh.ublock else (self.vals, None)
-                mgh.setup(_stream(), vals=hv, op_vals=op)
+                mgh.setup(_stream(), vals=hv, op_vals=op, coarse=full)
                 entry[1] = self.jac_version
+                if full:
+                    entry[2], entry[4] = self.step_id, False
             V, z, w, dinv, part, dh = self.gmres_workspace(self.nb, fp32, restart)
-            return mgh.gmres(b, x, V, z, w, part, dh, restart, tol, maxiter, _stream())
+            its, rel = mgh.gmres(b, x, V, z, w, part, dh, restart, tol, maxiter, _stream())
+            if full:
+                entry[3] = its
+            elif its > 3 * entry[3] + 20:
+                entry[4] = True
+            return its, rel
         if method == 'gmres':
             V, z, w, dinv, part, dh = self.gmres_workspace(self.nb, fp32, restart)
             L.check(lib.gm_gmres(self.nb, int(fp32), self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx),

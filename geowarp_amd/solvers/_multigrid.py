@@ -127,10 +127,12 @@ class Multigrid:
     def n_levels(self):
         return len(self.dims)
 
-    def setup(self, stream, vals=None, op_vals=None):
+    def setup(self, stream, vals=None, op_vals=None, coarse=True):
         """Point level 0 at the current Jacobian (or at `vals`, a matrix in its
         pattern the hierarchy is built from, with op_vals then the Krylov
-        operator) and build the coarse operators."""
+        operator) and build the coarse operators; with coarse=False only level 0
+        (its block-diagonal inverses and FP16 copy) is refreshed and the coarse
+        operators of the last full setup are kept (same pattern and free dofs)."""
         eng = self.eng
         lv = self.levels[0]
         self.hier_vals = eng.vals if vals is None else vals
@@ -158,8 +160,10 @@ class Multigrid:
                     'gm_up_setup')
         else:
             lv.vals, lv.freem = self.hier_vals.data_ptr(), eng.freem.data_ptr()
-            L.check(L.lib().gm_mg_setup(self.dim, self.nb, int(self.fp32), self.n_levels, self.levels,
-                                        L.ptr(self.ap), stream), 'gm_mg_setup')
+            L.check(L.lib().gm_mg_setup(self.dim, self.nb, int(self.fp32), self.n_levels if coarse else 1,
+                                        self.levels, L.ptr(self.ap), stream), 'gm_mg_setup')
+            if not coarse:
+                return
         if self.dense:
             # exact coarsest solve: pseudo-inverse of the (small) coarsest operator, formed on the host
             k = self._keep[-1]

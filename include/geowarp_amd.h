@@ -79,6 +79,14 @@ int gm_particle_stage(const gm_grid* g, const gm_stage_args* a, hipStream_t stre
 int gm_node_residual(const gm_grid* g, const gm_bins* bins, const double* Y, const uint8_t* freem, double* rhs,
                      int coupled, hipStream_t stream);
 
+/* The same residual for the 3D u-p form (mpm_utils.py:1020-1294) summed per particle
+   bin (the 27 nodes of each bin, its particles in bin order) into `work` (at least
+   gm_residual_work_bytes(g): n_nodes x 108 doubles), then per node over the <= 27 bins
+   holding it in a fixed order; masked like gm_node_residual. */
+size_t gm_residual_work_bytes(const gm_grid* g);
+int gm_node_residual_binned(const gm_grid* g, const gm_bins* bins, const double* Y, const uint8_t* freem,
+                            double* rhs, void* work, size_t work_bytes, hipStream_t stream);
+
 /* Node-gather Jacobian into the pattern: K = -dR/du between free & active
    dofs, identity on the others (reference sign and masking:
    sparse_differentiation.py:253-262, mpm_utils.py:1299-1341).  D and vals may
@@ -97,6 +105,20 @@ size_t gm_node_jacobian_work_bytes(const gm_grid* g, int tangent_fp32);
 int gm_node_jacobian_binned(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, const void* D,
                             int tangent_fp32, const uint8_t* freem, void* vals, int vals_fp32, void* work,
                             size_t work_bytes, hipStream_t stream);
+/* Its two phases, callable separately (gm_node_jacobian_binned is the pair): the
+   per-bin products into `work`, and the per-block gather from `work` into vals. */
+int gm_jacobian_bins(const gm_grid* g, const gm_bins* bins, const void* D, int tangent_fp32, void* work,
+                     size_t work_bytes, hipStream_t stream);
+int gm_jacobian_gather(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, int tangent_fp32,
+                       const void* work, const uint8_t* freem, void* vals, int vals_fp32, hipStream_t stream);
+
+/* 3D consistent-mass matrix M_IJ = sum_p m w_I w_J (mpm_utils.py:327-383) in the
+   pattern, assembled per particle bin into `work` (at least gm_mass_work_bytes(g):
+   n_nodes x 729 doubles) and gathered per row in a fixed order; gm_p2g then
+   leaves mvals null and fills only the activation flags and the right-hand side. */
+size_t gm_mass_work_bytes(const gm_grid* g);
+int gm_mass_binned(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, double p_mass, double* mvals,
+                   void* work, size_t work_bytes, hipStream_t stream);
 
 /* Preconditioner copy of the 3D u-p Newton matrix: adds coef * w_I * w_J to
    the zz entry of block (I, J) for every
