@@ -22,12 +22,16 @@ with the mixed/FP64 time ratio over the compared steps and the field errors at
 the last of them.  value = Newton iterations per second of the mixed form over
 the timed load steps; every timed step must converge, or value is null.
 
-roofline: the dominant kernel of the timed form, timed live with HIP events on
-its launch stream over repeated launches on the last Jacobian of the timed
-region, with its algorithmic bytes per launch (stated in the entry); traffic is
-the counted HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
-WRITE_SIZE passes (profiles/), or null.  roofline_krylov_spmv: the FP32
-block-CSR SpMV of the Krylov operator, likewise.
+roofline: the dominant kernel of the timed form, the per-bin phase of the
+Jacobian assembly (k_jac_bin: the 27 x 27 block products of every particle bin
+on the matrix cores), timed live with HIP events on its launch stream over
+repeated launches on the last tangent of the timed region; achieved =
+algorithmic flops per launch (2 x 60 per particle and block pair x 729 pairs)
+over that time, against the dense FP32 matrix peak; traffic is the counted HBM
+bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+(profiles/), or null.  roofline_smoother: the level-0 FP16 multigrid smoother
+sweep (k_level), and roofline_krylov_spmv: the FP32 block-CSR SpMV of the
+Krylov operator, likewise against the HBM peak with their algorithmic bytes.
 cpu_baseline: one Newton iteration of the same workload's first load step on its
 16 x 16 x 8-cell sub-block, on the host: the C++/OpenMP FP64 restatement of the
 assembly (oracle/cpu_port.cpp: residual + exact Jacobian, every core this
@@ -52,6 +56,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'Newton steps/s + adjoint grads/s, mixed vs FP64 at 1e-8 rel. residual, 1 GPU'
 HBM_PEAK_GBS = 8000.0
+MFMA_FP32_PEAK_TFS = 157.3  # MI355X dense FP32 matrix
+MFMA_FP64_PEAK_TFS = 78.6
 
 
 def parse():
@@ -72,6 +78,8 @@ def parse():
                     help="indenter penalty: 'physical' (pushes penetrating soil down) or 'reference' (the "
                          "reference's sign, a negative stiffness under the indenter)")
     ap.add_argument('--substeps', type=int, default=4, help='levels of load-step halving on Newton failure')
+    ap.add_argument('--mg-lag', type=int, default=0,
+                    help='1: keep the coarse multigrid operators of a load step\'s first Newton correction')
     ap.add_argument('--budget', type=float, default=540.0,
                     help='wall-clock seconds for the whole run; the optional blocks are skipped past it')
     ap.add_argument('--no-cpu', action='store_true')
@@ -97,6 +105,7 @@ def build_solver(cells, precision, args, tol=1e-8):
                      dp_apex=args.dp_apex, shape_factor=args.shape_factor, footing_start=args.footing_start,
                      penalty_form=args.penalty_form)
     cfg['iteration_dict']['substeps'] = args.substeps
+    cfg['iteration_dict']['mg_lag'] = bool(args.mg_lag)
     s = P.build_solver(gs, cfg, device='cuda:0')
     cfg['particle_flags'](s.x_particles, s.particle_traction_flag_array, s.particle_pressure_boundary_flag_array)
     s.line_search = True
@@ -228,6 +237,45 @@ def smoother_roofline(s, reps=50):
     bytes_ = mat + eng.n_nodes * nb * nb * sz + 3 * n * sz  # + dinv, x, b, out
     return dict(kernel='k_level<%d,16,Jacobi sweep,%s>' % (nb, 'FP16 copy' if lo else ('float' if fp32 else 'double')),
                 avg_us=avg_ms * 1e3, bytes_per_launch=bytes_, achieved_GBs=bytes_ / (avg_ms * 1e-3) / 1e9)
+
+
+def jacobian_roofline(s, reps=20):
+    """Time the per-bin phase of the binned Jacobian assembly (k_jac_bin through
+    gm_jacobian_bins: every bin's 27 x 27 block products on the matrix cores) on the
+    last tangent of the timed region, with HIP events on the launch stream.
+    Algorithmic flops per launch: 2 x 60 per (particle, block pair) -- 4 rows x 15
+    non-zero (column dof, shape component) terms -- x 729 block pairs per particle."""
+    import torch
+    from geowarp_amd import _lib as L
+    eng = s.engine
+    fp32 = eng._tangent_fp32()
+    lib = L.lib()
+    g, b = L.ctypes.byref(eng.grid), L.ctypes.byref(eng.bins)
+    need = int(lib.gm_node_jacobian_work_bytes(g, int(fp32)))
+    w = eng._bins_workspace(need)
+    stream = L.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def launch():
+        L.check(lib.gm_jacobian_bins(g, b, L.ptr(eng.D), int(fp32), L.ptr(w), need, stream), 'gm_jacobian_bins')
+
+    for _ in range(3):
+        launch()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        launch()
+    e1.record()
+    torch.cuda.synchronize()
+    avg_ms = e0.elapsed_time(e1) / reps
+    n_p = int(s.n_particles)
+    flops = 2.0 * 60 * 729 * n_p
+    nonempty = int((eng.cell_start[1:] != eng.cell_start[:-1]).sum().item())
+    sz = 4 if fp32 else 8
+    bytes_ = n_p * (17 * 15 * sz + 48) + nonempty * 27 * 27 * 16 * sz
+    return dict(kernel='k_jac_bin<%s> (gm_jacobian_bins)' % ('float' if fp32 else 'double'), avg_us=avg_ms * 1e3,
+                flops_per_launch=flops, achieved_TFs=flops / (avg_ms * 1e-3) / 1e12, bytes_per_launch=bytes_,
+                achieved_GBs=bytes_ / (avg_ms * 1e-3) / 1e9, nonempty_bins=nonempty)
 
 
 def adjoint_bench(steps, total=30, every=5, n_specimens=64, krylov='mg'):
@@ -405,12 +453,12 @@ def c3_bench(steps):
 
 
 def measured_traffic(cells, material, kernel='k_spmv'):
-    """HBM bytes per launch of a level-0 kernel (k_spmv: the FP32 SpMV; k_level: the
-    FP16 smoother sweep) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    (profiles/<kernel>_c4_indentation*), scaled by the calibration kernel; None for
-    other workloads."""
+    """HBM bytes per launch of a kernel (k_spmv: the FP32 SpMV; k_level: the FP16
+    smoother sweep; k_jac_bin: the per-bin Jacobian products) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/<kernel>_c4_indentation*),
+    scaled by the calibration kernel; None for other workloads."""
     path = os.path.join(ROOT, 'profiles', '%s_c4_indentation_traffic.json' % kernel)
-    if tuple(cells) != (64, 64, 32) or material != 'Neo-Hookean' or not os.path.exists(path):
+    if tuple(cells) != (64, 64, 32) or material != 'Neo-Hookean' or not os.path.isfile(path):
         return None
     with open(path) as f:
         return json.load(f)['traffic_bytes_per_launch']
@@ -575,14 +623,24 @@ def main():
         del s64
         torch.cuda.empty_cache()
 
-    # roofline of the dominant kernel of the timed form and of the Krylov SpMV (last Jacobian)
+    # roofline of the dominant kernel of the timed form (the per-bin Jacobian products), of the
+    # level-0 smoother sweep and of the Krylov SpMV (last Jacobian of the timed region)
+    jr = jacobian_roofline(smx)
     smo = smoother_roofline(smx)
     roof = spmv_roofline(smx)
-    out['roofline'] = {'bound': 'hbm', 'achieved': round(smo['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                       'frac': round(smo['achieved_GBs'] / HBM_PEAK_GBS, 4),
-                       'traffic': measured_traffic(cells, args.material, 'k_level'),
-                       'kernel': smo['kernel'], 'avg_us': round(smo['avg_us'], 2),
-                       'algorithmic_bytes': smo['bytes_per_launch']}
+    peak = MFMA_FP32_PEAK_TFS if smx.engine._tangent_fp32() else MFMA_FP64_PEAK_TFS
+    out['roofline'] = {'bound': 'mfma', 'achieved': round(jr['achieved_TFs'], 2), 'peak': peak, 'unit': 'TFLOP/s',
+                       'frac': round(jr['achieved_TFs'] / peak, 4),
+                       'traffic': measured_traffic(cells, args.material, 'k_jac_bin'),
+                       'kernel': jr['kernel'], 'avg_us': round(jr['avg_us'], 2),
+                       'algorithmic_flops': jr['flops_per_launch'],
+                       'bytes_per_launch': jr['bytes_per_launch'],
+                       'hbm_frac_of_those_bytes': round(jr['achieved_GBs'] / HBM_PEAK_GBS, 4)}
+    out['roofline_smoother'] = {'bound': 'hbm', 'achieved': round(smo['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS,
+                                'unit': 'GB/s', 'frac': round(smo['achieved_GBs'] / HBM_PEAK_GBS, 4),
+                                'traffic': measured_traffic(cells, args.material, 'k_level'),
+                                'kernel': smo['kernel'], 'avg_us': round(smo['avg_us'], 2),
+                                'algorithmic_bytes': smo['bytes_per_launch']}
     out['roofline_krylov_spmv'] = {'bound': 'hbm', 'achieved': round(roof['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS,
                                    'unit': 'GB/s', 'frac': round(roof['achieved_GBs'] / HBM_PEAK_GBS, 4),
                                    'traffic': measured_traffic(cells, args.material),
