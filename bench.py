@@ -78,8 +78,14 @@ def parse():
                     help="indenter penalty: 'physical' (pushes penetrating soil down) or 'reference' (the "
                          "reference's sign, a negative stiffness under the indenter)")
     ap.add_argument('--substeps', type=int, default=4, help='levels of load-step halving on Newton failure')
-    ap.add_argument('--mg-lag', type=int, default=0,
-                    help='1: keep the coarse multigrid operators of a load step\'s first Newton correction')
+    ap.add_argument('--gravity', type=float, default=0.0,
+                    help='gravity magnitude (m/s^2; the reference driver and the default: 0, a stress-free block)')
+    ap.add_argument('--forcing-max', type=float, default=None,
+                    help='cap of the mixed form\'s Eisenstat-Walker inner tolerance (the solver\'s default: '
+                         '1e-2 for elastic materials, 1e-4 for plastic ones)')
+    ap.add_argument('--mg-lag', type=int, default=1,
+                    help='1: keep the coarse multigrid operators of a load step\'s first Newton correction; '
+                         '0: rebuild them for every correction')
     ap.add_argument('--budget', type=float, default=540.0,
                     help='wall-clock seconds for the whole run; the optional blocks are skipped past it')
     ap.add_argument('--no-cpu', action='store_true')
@@ -103,9 +109,11 @@ def build_solver(cells, precision, args, tol=1e-8):
     cfg = P.indent3d(n_cells=cells, dx=dx, total_steps=40, depth=0.05, footing=0.2 * cells[0] / 64.0,
                      material=args.material, precision=precision, tol=tol, n_iter=12, krylov=args.krylov,
                      dp_apex=args.dp_apex, shape_factor=args.shape_factor, footing_start=args.footing_start,
-                     penalty_form=args.penalty_form)
+                     penalty_form=args.penalty_form, gravity=args.gravity)
     cfg['iteration_dict']['substeps'] = args.substeps
     cfg['iteration_dict']['mg_lag'] = bool(args.mg_lag)
+    if args.forcing_max is not None:
+        cfg['iteration_dict']['forcing_max'] = args.forcing_max
     s = P.build_solver(gs, cfg, device='cuda:0')
     cfg['particle_flags'](s.x_particles, s.particle_traction_flag_array, s.particle_pressure_boundary_flag_array)
     s.line_search = True
@@ -558,6 +566,8 @@ def main():
                                                         args.footing_start, args.warmup,
                                                         args.warmup + args.steps - 1)),
                      'nodes': smx.n_nodes, 'dofs': smx.n_matrix_size, 'substep_levels': args.substeps}
+    if args.gravity:
+        out['config']['gravity'] = args.gravity
     out['warmup_steps'] = summarize(warm, args.warmup)
     if not out['warmup_steps']['all_converged']:
         out['value'] = None

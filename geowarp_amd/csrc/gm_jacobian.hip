@@ -106,15 +106,18 @@ __global__ void __launch_bounds__(256, 2) k_jac_bin(gm_grid g, const double* x_s
   constexpr int KST = 5 * PG / 4;  // K steps of 4 per group
   __shared__ TC AXT[PG][3][9];
   // E[p][I][r][z] plus a zero node row (I = 27) and a zero column (z = ZERO): every
-  // A operand is then an unconditional LDS read
-  __shared__ TC E[PG][28][4][NZ + 1];
+  // A operand is then an unconditional LDS read.  A node's 4 x 16 entries are padded
+  // to 65 words, so that the 16 (I, r) rows an MFMA reads at once fall in distinct banks
+  constexpr int ES = 4 * (NZ + 1) + 1;
+  __shared__ TC E_[PG][28][ES];
+#define E(p, I, r, z) E_[p][I][(r) * (NZ + 1) + (z)]
   __shared__ TC BT[PG][5][32];  // B operand g_p[J][beta], J padded to 32 with zeros
   const int b = blockIdx.x;
   const int ks = cell_start[b], ke = cell_start[b + 1];
   if (ks == ke) return;  // uniform across the workgroup
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int q = t; q < PG * 4 * (NZ + 1); q += 256) E[q / (4 * (NZ + 1))][27][(q / (NZ + 1)) % 4][q % (NZ + 1)] = 0;
-  for (int q = t; q < PG * 27 * 4; q += 256) E[q / 108][(q / 4) % 27][q % 4][ZERO] = 0;
+  for (int q = t; q < PG * 4 * (NZ + 1); q += 256) E(q / (4 * (NZ + 1)), 27, (q / (NZ + 1)) % 4, q % (NZ + 1)) = 0;
+  for (int q = t; q < PG * 27 * 4; q += 256) E(q / 108, (q / 4) % 27, q % 4, ZERO) = 0;
   typedef typename Mfma<TC>::acc ACC;
   ACC acc[4][4];
 #pragma unroll
@@ -209,7 +212,7 @@ __global__ void __launch_bounds__(256, 2) k_jac_bin(gm_grid g, const double* x_s
         for (int iy = 0; iy < 3; ++iy) {
           const TC U = wy[iy] * u + gy[iy] * v, Q = wy[iy] * qx, A2 = ay[iy] * a;
 #pragma unroll
-          for (int iz = 0; iz < 3; ++iz) E[p][ix + 3 * iy + 9 * iz][r][z] = wz[iz] * U + gz[iz] * Q + az[iz] * A2;
+          for (int iz = 0; iz < 3; ++iz) E(p, ix + 3 * iy + 9 * iz, r, z) = wz[iz] * U + gz[iz] * Q + az[iz] * A2;
         }
       }
     }
@@ -232,7 +235,7 @@ __global__ void __launch_bounds__(256, 2) k_jac_bin(gm_grid g, const double* x_s
         const int I = min(mt * 4 + (Mfma<TC>::r_major ? (rt & 3) : (rt >> 2)), 27);
         const int r = Mfma<TC>::r_major ? (rt >> 2) : (rt & 3);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) av[s][c] = E[p][I][r][zc[c]];
+        for (int c = 0; c < 4; ++c) av[s][c] = E(p, I, r, zc[c]);
       }
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
@@ -258,6 +261,8 @@ __global__ void __launch_bounds__(256, 2) k_jac_bin(gm_grid g, const double* x_s
     }
   }
 }
+
+#undef E
 
 // One workgroup per block row I, four lanes per block (one per block row r).  The
 // row's 27 candidate bins (b = I - o, -1 when outside the grid or empty) are read
@@ -547,21 +552,17 @@ extern "C" int gm_jacobian_gather(const gm_grid* g, const gm_bins* b, const gm_p
   GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_jacobian_gather: the 3D u-p form only");
   const int n = g->n_nodes;
   if (n == 0) return 0;
+#define GM_GATHER(TCC, TVV)                                                                              \
+  k_jac_gather<TCC, TVV><<<n, 512, 0, s>>>(*g, b->cell_start, (const TCC*)work, freem, pat->row_ptr, \
+                                           pat->row_mask, (TVV*)vals)
   if (tangent_fp32) {
-    if (vals_fp32)
-      k_jac_gather<float, float><<<n, 512, 0, s>>>(*g, b->cell_start, (const float*)work, freem, pat->row_ptr,
-                                                   pat->row_mask, (float*)vals);
-    else
-      k_jac_gather<float, double><<<n, 512, 0, s>>>(*g, b->cell_start, (const float*)work, freem, pat->row_ptr,
-                                                    pat->row_mask, (double*)vals);
+    if (vals_fp32) GM_GATHER(float, float);
+    else GM_GATHER(float, double);
   } else {
-    if (vals_fp32)
-      k_jac_gather<double, float><<<n, 512, 0, s>>>(*g, b->cell_start, (const double*)work, freem, pat->row_ptr,
-                                                    pat->row_mask, (float*)vals);
-    else
-      k_jac_gather<double, double><<<n, 512, 0, s>>>(*g, b->cell_start, (const double*)work, freem, pat->row_ptr,
-                                                     pat->row_mask, (double*)vals);
+    if (vals_fp32) GM_GATHER(double, float);
+    else GM_GATHER(double, double);
   }
+#undef GM_GATHER
   GM_LAUNCH_CHECK();
   return 0;
 }

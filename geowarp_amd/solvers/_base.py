@@ -181,7 +181,7 @@ class _QuasiStaticBase:
         n_grid = (self.n_grid_x, self.n_grid_y) if self.dim == 2 else (self.n_grid_x, self.n_grid_y, self.n_grid_z)
         self.engine = Engine(self.dim, self.coupled, n_grid, spacing, device=self.device, precision=self.precision)
         self.engine.mixed_assembly = it.get('mixed_assembly', 'fp32')
-        self.engine.mg_lag = bool(it.get('mg_lag', False))
+        self.engine.mg_lag = bool(it.get('mg_lag', True))
         self.engine.mg_options = mg_opts
         dev = self.device
         self.dofStruct = DofStruct(self.n_matrix_size, dev)
