@@ -94,6 +94,7 @@ _SIGS = {
     'gm_jacobian_gather': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern),
                                    c_int, _P, _P, _P, c_int, _P]),
     'gm_mass_work_bytes': (ctypes.c_size_t, [ctypes.POINTER(gm_grid)]),
+    'gm_stage_de_3d': (c_int, [c_int, _P, _P, _P, _P, _P, _P, c_double, c_double, _P, _P]),
     'gm_residual_work_bytes': (ctypes.c_size_t, [ctypes.POINTER(gm_grid)]),
     'gm_node_residual_binned': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), _P, _P, _P, _P,
                                         ctypes.c_size_t, _P]),

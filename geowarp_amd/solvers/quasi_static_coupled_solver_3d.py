@@ -12,7 +12,9 @@ G2P and the lp update run at the end of ``newton_iter`` as in the reference
 import numpy as np
 import torch
 
+from .. import _lib as L
 from ._base import _QuasiStaticBase, init_particles_rectangle
+from ._engine import _stream
 
 TARGET_FORCE = 13567.27337196  # qc3d:52, "from E = 1e4 kPa"
 
@@ -62,6 +64,11 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
         self.engine.penalty_pc = iteration_dict.get('penalty_pc', True)
         # adaptive load-step splitting on Newton failure (0: off, the reference's behaviour)
         self.max_substeps = int(iteration_dict.get('substeps', 0))
+        # d loss / d E of every converged load step at its fixed initial state (the adjoint of
+        # the load step), in E_grad_step; the explicit part alone is E_grad_step_explicit
+        self.step_sensitivity = bool(iteration_dict.get('step_sensitivity', False))
+        self.E_grad_step = None
+        self.E_grad_step_explicit = None
         self.substep_count = 0
         self.step_newton_iterations = 0
         dev = self.device
@@ -169,6 +176,8 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
                                     self.footing_region_max_y), height=args.footing_height, pen_par=args.pen_par,
                            lp_max=float(self.GIMP_lp.max()))
         self._newton(current_step, total_steps, self._new, self._old, args, after_residual=after_residual)
+        if self.step_sensitivity:
+            self._step_E_sensitivity(total_steps, args)
         eng.g2p(self.x_particles, self.GIMP_lp, self._new, self._old, self.deformation_gradient_total_old,
                 self.deformation_gradient_total_new, self.left_Cauchy_Green_old, self.left_Cauchy_Green_new,
                 self.plastic_strain_old, self.plastic_strain_new, self.particle_pressure_array)
@@ -176,6 +185,70 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
                       (self.domain_min_x, self.domain_min_y, self.domain_min_z),
                       (self.domain_max_x, self.domain_max_y, self.domain_max_z))
         self.indentation_force_list = np.append(self.indentation_force_list, float(self.indentation_force.item()))
+
+    def _step_E_sensitivity(self, total_steps, args, tol=1e-12):
+        """Adjoint of the converged load step for the indentation loss L = (F/S - target)^2
+        (qc3d:45-52) with respect to Young's modulus, at the step's fixed initial state:
+        dL/dE = dL/dE|_u + lambda . dR/dE with K^T lambda = dL/du (K = -dR/du on the free
+        dofs).  dL/dE|_u is the closed form of the explicit gradient (the force is linear in
+        E through the penalty, qc3d:323-325); dL/du = dL/dF dF/du with dF/du_{J,z} =
+        -pen_par sum_p w_J over the penalised particles; dR/dE gathers the particle rows of
+        gm_stage_de_3d.  Both vectors are gathered by the residual kernel itself with a
+        substituted particle stress vector.  FP64 form and Neo-Hookean skeleton only (there
+        dR/dE follows from the stress being proportional to E at fixed deformation)."""
+        eng = self.engine
+        if eng.precision != 'fp64' or self.material_name != 'Neo-Hookean':
+            raise L.GMError('the load-step sensitivity needs the FP64 form and the Neo-Hookean skeleton')
+        x, lp = self.x_particles, self.GIMP_lp
+        n, NY = self.n_particles, eng.NY
+        # stage, residual and Jacobian at the converged unknowns (after the last update)
+        eng.stage(args, want_tangent=True)
+        eng.residual(x, lp)
+        eng.sum_into(eng.pen_force, 1)
+        F = float(eng.scal[1].item())
+        eng.jacobian(x, lp)
+        S = abs(self.footing_incr_disp) * float(total_steps)
+        dLdF = 2.0 * (F / S - TARGET_FORCE) / S
+        E = float(self.youngs_modulus)
+        h = float(args.footing_height)
+        inside = ((x[:, 2] >= h) & (x[:, 0] > self.footing_region_min_x) & (x[:, 0] < self.footing_region_max_x)
+                  & (x[:, 1] > self.footing_region_min_y) & (x[:, 1] < self.footing_region_max_y))
+        order = eng.sorted[:n].long()
+        Y = eng.Y
+        try:
+            yg = torch.zeros(n * NY, dtype=torch.float64, device=self.device)
+            yg.view(n, NY)[:, 11] = inside[order].to(torch.float64) * (-float(args.pen_par) * dLdF)
+            eng.Y = yg
+            eng.residual(x, lp)
+            g = eng.rhs.clone()
+            dY = torch.empty(n * NY, dtype=torch.float64, device=self.device)
+            L.check(L.lib().gm_stage_de_3d(n, L.ptr(eng.sorted), L.ptr(self.deformation_gradient_total_old),
+                                        L.ptr(self.deformation_gradient_total_new),
+                                        L.ptr(self.particle_Cauchy_stress_array), L.ptr(eng.pen_force), L.ptr(Y),
+                                        float(self.p_vol), E, L.ptr(dY), _stream()), 'gm_stage_de_3d')
+            eng.Y = dY
+            eng.residual(x, lp)
+            rE = eng.rhs.clone()
+        finally:
+            eng.Y = Y
+        # K^T lambda = g on the FP64 Jacobian (block transpose; a full hierarchy for K^T)
+        nb = eng.nb
+        vt = torch.empty(eng.nnzb * nb * nb, dtype=torch.float64, device=self.device)
+        L.check(L.lib().gm_bsr_transpose(nb, 0, eng.n_nodes, L.ptr(eng.row_ptr), L.ptr(eng.col_idx), L.ptr(eng.vals),
+                                         L.ptr(vt), _stream()), 'gm_bsr_transpose')
+        vals, lag = eng.vals, eng.mg_lag
+        eng.vals, eng.mg_lag = vt, False
+        eng.jac_version += 1
+        lam = torch.zeros_like(g)
+        try:
+            its, rel = eng.krylov(g, lam, False, tol, self.linear_maxiter, self.krylov_method if
+                                  self.krylov_method in ('mg', 'gmres') else 'gmres', self.krylov_restart)
+        finally:
+            eng.vals, eng.mg_lag = vals, lag
+            eng.jac_version += 1
+        self.E_grad_step_explicit = dLdF * F / E
+        self.E_grad_step = self.E_grad_step_explicit + float(torch.dot(lam, rE).item())
+        self.sensitivity_stats = dict(krylov_iters=its, relres=rel, force=F)
 
     def prepare_step(self, current_step, total_steps):
         """reset, boundary callback, P2G and consistent-mass projection (qc3d:397-417)."""

@@ -112,6 +112,17 @@ int gm_jacobian_bins(const gm_grid* g, const gm_bins* bins, const void* D, int t
 int gm_jacobian_gather(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, int tangent_fp32,
                        const void* work, const uint8_t* freem, void* vals, int vals_fp32, hipStream_t stream);
 
+/* Load-step sensitivity of the 3D u-p Neo-Hookean form to Young's modulus: per
+   particle (bin-sorted row kq, particle sorted[kq]) the derivative of its stress
+   vector Y with respect to E at fixed nodal unknowns, dP/dE = -V A^T sigma / E,
+   dT_z/dE = -pen_force / E, dS/dE = -S / E, zero elsewhere (the stress and the
+   penalty are proportional to E, the PPP coefficient to 1/E).  Gathered like the
+   residual it is dR/dE of the adjoint dL/dE = dL/dE|_u + lambda^T dR/dE.  The
+   reference differentiates only the explicit term (qc3d:323-325); no counterpart
+   for the rest. */
+int gm_stage_de_3d(int n, const int* sorted, const double* F_old, const double* F_new, const double* sigma,
+                const double* pen_force, const double* Y, double p_vol, double E, double* dY, hipStream_t stream);
+
 /* 3D consistent-mass matrix M_IJ = sum_p m w_I w_J (mpm_utils.py:327-383) in the
    pattern, assembled per particle bin into `work` (at least gm_mass_work_bytes(g):
    n_nodes x 729 doubles) and gathered per row in a fixed order; gm_p2g then
