@@ -355,9 +355,13 @@ extern "C" int gm_bsr_pattern(const gm_grid* g, const gm_bins* b, gm_pattern* pa
 }
 
 // ================================================================== particle stage
-template <int DIM, bool COUPLED, typename TD>
+// NHONLY: an instance for the Neo-Hookean skeleton alone, compiled without the
+// Hencky-family return maps (eigen-decomposition, local Newton solves) and so with
+// a fraction of the general instance's registers.
+template <int DIM, bool COUPLED, typename TD, bool NHONLY = false>
 __global__ void __launch_bounds__(128) k_particle_stage(gm_grid g, gm_stage_args a) {
   typedef Lay<DIM, COUPLED> L;
+  const bool neo_hookean = NHONLY || a.mat_kind == MAT_NEOHOOKEAN;
   int kq = blockIdx.x * blockDim.x + threadIdx.x;  // bin-sorted position: Y/D row
   if (kq >= a.n_particles) return;
   int p = a.sorted ? a.sorted[kq] : kq;               // particle id: state arrays
@@ -443,7 +447,7 @@ __global__ void __launch_bounds__(128) k_particle_stage(gm_grid g, gm_stage_args
   double alpha_o = a.alpha_old ? a.alpha_old[p] : 0.0;
   double alpha_n = alpha_o;
   M3 sig;
-  if (a.mat_kind == MAT_NEOHOOKEAN) {
+  if (neo_hookean) {
     // sigma = mu/J (F F^T - I) + lambda/J ln J I (mpm_utils.py:915)
     M3 FFt = m3_mul(Fn, m3_T(Fn));
     double lnJ = log(J);
@@ -456,7 +460,7 @@ __global__ void __launch_bounds__(128) k_particle_stage(gm_grid g, gm_stage_args
         tau.a[i][j] = sig.a[i][j] * J;
       }
     bn = btr;
-  } else {
+  } else if constexpr (!NHONLY) {
     hencky_family(btr, mp, alpha_o, ss, tau, bn, alpha_n);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -561,7 +565,7 @@ __global__ void __launch_bounds__(128) k_particle_stage(gm_grid g, gm_stage_args
       AtM.a[i][j] = s;
     }
   M3 Bnh;  // F_old F^T for Neo-Hookean, b_old f^T for the Hencky family
-  if (a.mat_kind == MAT_NEOHOOKEAN) Bnh = m3_mul(Fo, m3_T(Fn));
+  if (neo_hookean) Bnh = m3_mul(Fo, m3_T(Fn));
   else Bnh = m3_mul(bo, m3_T(f));
   double lnJ = log(J), lnJo = log(Jo);
   // A^T A, A g, A^T g, A^T A g (mass-balance terms)
@@ -591,7 +595,7 @@ __global__ void __launch_bounds__(128) k_particle_stage(gm_grid g, gm_stage_args
       int z = aa * DIM + bb;  // column of dH_ab
       double dJ = J * A.a[bb][aa];
       M3 dtau;
-      if (a.mat_kind == MAT_NEOHOOKEAN) {
+      if (neo_hookean) {
         // d tau = mu (dH B + B^T dH^T) + lambda tr(A dH) I, B = F_old F^T
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -702,19 +706,28 @@ extern "C" int gm_particle_stage(const gm_grid* g, const gm_stage_args* a, hipSt
   if (n == 0) return 0;
   GM_REQUIRE(!a->want_tangent || a->D, "gm_particle_stage: tangent requested without D");
   int blk = gm_blocks(n, 128);
+  const bool nh = a->mat_kind == MAT_NEOHOOKEAN;
+#define GM_STAGE(D, C)                                                                               \
+  do {                                                                                              \
+    if (nh) {                                                                                       \
+      if (a->tangent_fp32) k_particle_stage<D, C, float, true><<<blk, 128, 0, s>>>(*g, *a);         \
+      else k_particle_stage<D, C, double, true><<<blk, 128, 0, s>>>(*g, *a);                        \
+    } else {                                                                                        \
+      if (a->tangent_fp32) k_particle_stage<D, C, float, false><<<blk, 128, 0, s>>>(*g, *a);        \
+      else k_particle_stage<D, C, double, false><<<blk, 128, 0, s>>>(*g, *a);                       \
+    }                                                                                               \
+  } while (0)
   if (g->dim == 2 && !a->coupled) {
-    if (a->tangent_fp32) k_particle_stage<2, false, float><<<blk, 128, 0, s>>>(*g, *a);
-    else k_particle_stage<2, false, double><<<blk, 128, 0, s>>>(*g, *a);
+    GM_STAGE(2, false);
   } else if (g->dim == 2 && a->coupled) {
-    if (a->tangent_fp32) k_particle_stage<2, true, float><<<blk, 128, 0, s>>>(*g, *a);
-    else k_particle_stage<2, true, double><<<blk, 128, 0, s>>>(*g, *a);
+    GM_STAGE(2, true);
   } else if (g->dim == 3 && a->coupled) {
-    if (a->tangent_fp32) k_particle_stage<3, true, float><<<blk, 128, 0, s>>>(*g, *a);
-    else k_particle_stage<3, true, double><<<blk, 128, 0, s>>>(*g, *a);
+    GM_STAGE(3, true);
   } else {
     gm::set_error("gm_particle_stage: 3D single-phase is not a supported form");
     return -1;
   }
+#undef GM_STAGE
   GM_LAUNCH_CHECK();
   return 0;
 }
