@@ -93,6 +93,12 @@ _SIGS = {
                                  _P]),
     'gm_jacobian_gather': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern),
                                    c_int, _P, _P, _P, c_int, _P]),
+    'gm_jacobian_bins_variant': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), _P, c_int, c_int, _P,
+                                         ctypes.c_size_t, _P]),
+    'gm_jacobian_gather_variant': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins),
+                                           ctypes.POINTER(gm_pattern), c_int, c_int, _P, _P, _P, c_int, _P]),
+    'gm_jacobian_autotune': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern),
+                                     _P, c_int, _P, _P, c_int, _P, ctypes.c_size_t, c_int, _P, _P, _P, _P]),
     'gm_mass_work_bytes': (ctypes.c_size_t, [ctypes.POINTER(gm_grid)]),
     'gm_stage_de_3d': (c_int, [c_int, _P, _P, _P, _P, _P, _P, c_double, c_double, _P, _P]),
     'gm_residual_work_bytes': (ctypes.c_size_t, [ctypes.POINTER(gm_grid)]),
@@ -111,6 +117,8 @@ _SIGS = {
     'gm_update_lp': (c_int, [c_int, c_int, _P, _P, c_double, _P, ctypes.POINTER(c_double),
                              ctypes.POINTER(c_double), _P]),
     'gm_bicgstab': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, ctypes.POINTER(gm_krylov_ws), c_double, c_int,
+                            c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_double), _P]),
+    'gm_pcg': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, ctypes.POINTER(gm_krylov_ws), c_double, c_int,
                             c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_double), _P]),
     'gm_gmres': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, c_int, c_double, c_int,
                          ctypes.POINTER(c_int), ctypes.POINTER(c_double), _P]),
@@ -137,6 +145,7 @@ _SIGS = {
     'gm_penalty_pc': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern), _P, _P,
                               c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_double), c_double,
                               c_double, _P]),
+    'gm_krylov_graphs': (c_int, [c_int]),
     'gm_spmv_autotune': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, c_int, _P, _P, _P]),
     'gm_row_scale': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P]),
     'gm_up_setup': (c_int, [c_int, c_int, c_int, c_int, c_int, _P, _P, _P, _P, c_int, _P, _P, _P, _P]),

@@ -10,18 +10,21 @@
 // is a dense product (108 rows (I,r)) x (5 n_p) x (27 columns J): the matrix
 // cores take it 16x16x4 at a time, FP32 when the tangent is FP32 (mixed form),
 // FP64 otherwise.
-//   k_jac_bin     one 256-lane workgroup per bin: E_I for a group of particles
-//                 in LDS, 14 output tiles of 16x16 over the 4 waves, the
-//                 bin's 27 x 27 blocks written contiguously to a workspace
-//                 (n_nodes x 11664 values).
+//   k_jac_bin     a 256-lane workgroup per bin (or a persistent sweep over bins):
+//                 E_I for a group of particles in LDS, 14 output tiles of 16x16
+//                 over the 4 waves, the bin's 27 x 27 blocks written contiguously
+//                 to a workspace (n_nodes x 11664 values).
 //   k_jac_gather  one workgroup per block row I: every block (I, J) of the
 //                 pattern sums the <= 27 bins b = I - o holding both nodes in the
 //                 fixed order (oz, oy, ox); masks, sign and identity rows as in
-//                 the node-row kernels.
+//                 the node-row kernels (k_jac_gather_staged: the same sums from
+//                 chunks staged in LDS).
 // Both steps are deterministic (fixed accumulation order).  The workspace is
 // written once and read once per assembly, so the pair is bound by HBM on
 // 2 x n_bins x 11664 values plus the matrix itself.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "gm_common.h"
 #include "gm_device.h"
@@ -38,6 +41,10 @@ constexpr int NY = 17, NZ = 15;
 constexpr int Y_T = 9, Y_Q = 12, Y_M = 13, Y_S = 16;
 constexpr int Z_DU = 9, Z_P = 10, Z_PB = 11, Z_G = 12;
 constexpr long CB_BIN = 27L * 27L * 16L;  // values per bin: 27 x 27 blocks of 4 x 4
+// chunk of bin `bin`'s blocks (I_loc = o, J_loc = 0..26) in the workspace: bins one
+// after another, each its 27 chunks in I_loc order
+constexpr long CHUNK = 27L * 16L;
+__device__ __forceinline__ long chunk_at(int o, int bin) { return (long)bin * CB_BIN + o * CHUNK; }
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d4v __attribute__((ext_vector_type(4)));
@@ -95,51 +102,82 @@ __device__ __forceinline__ TC gcomp(const TC (*ax)[9], int J, int beta) {
 
 }  // namespace
 
-// One workgroup per bin (stencil base node b).  Workspace cb[b] = 27 x 27 blocks
-// (I_loc, J_loc), I_loc = o_x + 3 o_y + 9 o_z with node = b + o.  The bin's
-// particles are taken PG at a time (PG = 8 in FP32, 4 in FP64: the E table of a
-// group is 52 KB of LDS either way).
-template <typename TD, typename TC>
-__global__ void __launch_bounds__(256, 2) k_jac_bin(gm_grid g, const double* x_s, const double* lp_s,
-                                                    const int* cell_start, const TD* D, TC* cb) {
-  constexpr int PG = sizeof(TC) == 4 ? 8 : 4;
+// Per-bin products.  Workspace cb[b] = 27 x 27 blocks (I_loc, J_loc), I_loc = o_x +
+// 3 o_y + 9 o_z with node = b + o.  A workgroup takes bins b = blockIdx.x, +gridDim.x,
+// ... (one bin when the grid has one workgroup per bin, a persistent sweep when it is
+// smaller) and each bin's particles PG at a time: the E table of a group in LDS, its
+// products accumulated in the matrix-core registers, the bin's 27 x 27 blocks stored
+// after its last group.  The global loads of the next group (of this bin or of the
+// next one) are issued before the products of the current group when PF is set, so
+// their latency hides behind the matrix cores (at the cost of the registers that hold
+// them), else at the top of the group.
+// K order: a K step of the MFMA is one shape-function component beta and four
+// particles (lane group l / 16 = particle): the five components of a quad of
+// particles, then the next quad, whatever PG is.  After the
+// K loop unrolls, every A and B operand address is a per-lane base plus an immediate,
+// and the (c, beta) products that vanish (u columns: no w term except the penalty
+// on u_z, no w_avg term) are not issued.  Every (PG, OCC, PF, grid) variant adds the
+// same products in the same order, so all of them write the same bits.
+template <typename TD, typename TC, int PG, int OCC, bool PF>
+__global__ void __launch_bounds__(256, OCC) k_jac_bin(gm_grid g, const double* x_s, const double* lp_s,
+                                                      const int* cell_start, int n_bins, const TD* D, TC* cb) {
+  static_assert(PG % 4 == 0, "particle groups are whole K steps");
   constexpr int KST = 5 * PG / 4;  // K steps of 4 per group
-  __shared__ TC AXT[PG][3][9];
-  // E[p][I][r][z] plus a zero node row (I = 27) and a zero column (z = ZERO): every
-  // A operand is then an unconditional LDS read.  A node's 4 x 16 entries are padded
-  // to 65 words, so that the 16 (I, r) rows an MFMA reads at once fall in distinct banks
-  constexpr int ES = 4 * (NZ + 1) + 1;
-  __shared__ TC E_[PG][28][ES];
-#define E(p, I, r, z) E_[p][I][(r) * (NZ + 1) + (z)]
-  __shared__ TC BT[PG][5][32];  // B operand g_p[J][beta], J padded to 32 with zeros
-  const int b = blockIdx.x;
-  const int ks = cell_start[b], ke = cell_start[b + 1];
-  if (ks == ke) return;  // uniform across the workgroup
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int q = t; q < PG * 4 * (NZ + 1); q += 256) E(q / (4 * (NZ + 1)), 27, (q / (NZ + 1)) % 4, q % (NZ + 1)) = 0;
-  for (int q = t; q < PG * 27 * 4; q += 256) E(q / 108, (q / 4) % 27, q % 4, ZERO) = 0;
+  constexpr int NI = 4 * NZ;       // (z, r) entries of a node row of E_p
+  // shape-function tables of a group, two buffers: with PF the next group's are
+  // computed (by wave 3, whose last row tile is empty) while the products run
+  __shared__ TC AXT_[2][PG][3][9];
+  // E_p[I][z][r] (r fastest), node rows 68 entries apart and particles 28 x 68 apart:
+  // the 32 (particle, I, r) entries half a wave reads at once fall in 32 distinct banks.
+  // Node row I = 27 is zero (the padding rows of the last row tile read it).
+  constexpr int EI = 68, EP = 28 * EI;
+  __shared__ TC E_[PG * EP];
+#define E(p, I, z, r) E_[(p) * EP + (I) * EI + (z) * 4 + (r)]
+  // B operand g_p[J][beta], J padded to 32 with zeros, particles 176 entries apart
+  constexpr int BB = 32, BP = 5 * BB + 16;
+  __shared__ TC BT_[PG * BP];
+#define BT(p, beta, J) BT_[(p) * BP + (beta) * BB + (J)]
+  int b = blockIdx.x, ks = 0, ke = 0;
+  for (; b < n_bins; b += gridDim.x) {  // first non-empty bin (uniform across the workgroup)
+    ks = cell_start[b];
+    ke = cell_start[b + 1];
+    if (ke > ks) break;
+  }
+  if (b >= n_bins) return;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // uniform across the wave
+  for (int q = t; q < PG * NI; q += 256) E(q / NI, 27, (q % NI) >> 2, q & 3) = 0;  // never written again
   typedef typename Mfma<TC>::acc ACC;
   ACC acc[4][4];
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[s][c] = ACC{0, 0, 0, 0};
-  // output tiles tt = wave + 4 s: row tile mt = tt / 2 (16 rows (I, r)), column tile
-  // nt = tt % 2 (16 columns J); tiles 14 and 15 (the fourth of waves 2 and 3) have
-  // I >= 28, read the zero row and are not stored
-  constexpr int NQ = (PG * 4 * NZ + 255) / 256;  // (p, r, z) items per thread
-  for (int p0 = ks; p0 < ke; p0 += PG) {
-    const int np = min(PG, ke - p0);
-    // every global load of the group is issued before the first barrier: the tangent
-    // entries of this thread's (p, r, z) items, then the positions and domain sizes
-    TC dv[NQ][5];
+  // output tiles tt = wave + 4 s: row tile mt = tt / 2 = wave / 2 + 2 s (16 rows (I, r)),
+  // column tile nt = tt % 2 = wave % 2 (16 columns J); waves 2 and 3 have no fourth
+  // tile (rows I >= 28)
+  const int lg = lane >> 4, rt = lane & 15;
+  const int r_l = Mfma<TC>::r_major ? (rt >> 2) : (rt & 3);
+  const int i_l = Mfma<TC>::r_major ? (rt & 3) : (rt >> 2);
+  int abase[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) abase[s] = lg * EP + min((wave >> 1) * 4 + 8 * s + i_l, 27) * EI + r_l;
+  const int bbase = lg * BP + (wave & 1) * 16 + rt;
+  constexpr int NQ = (PG * NI + 255) / 256;  // (p, z, r) items per thread
+  constexpr int GT0 = PF ? 192 : 0;           // first thread of the shape-function tables
+  static_assert(PG * 3 <= 64, "the tables of a group are one wave's work");
+  TC dv[NQ][5];
+  double xa = 0.0, la = 0.0;
+  // tangent entries of this thread's (p, z, r) items, position and domain size of
+  // particle (t - GT0) / 3 (axis (t - GT0) % 3) of the group starting at p0 (np particles)
+  auto load_group = [&](int p0, int np) {
 #pragma unroll
     for (int j = 0; j < NQ; ++j) {
       const int q = t + 256 * j;
-      const int p = q / (4 * NZ), rz = q % (4 * NZ), r = rz / NZ, z = rz % NZ;
+      const int p = q / NI, zr = q % NI, z = zr >> 2, r = zr & 3;
 #pragma unroll
       for (int i = 0; i < 5; ++i) dv[j][i] = (TC)0;
-      if (q < PG * 4 * NZ && p < np) {
+      if (q < PG * NI && p < np) {
         const TD* Dp = D + (long)(p0 + p) * NY * NZ;
         if (r < 3) {
           dv[j][0] = (TC)Dp[(0 * 3 + r) * NZ + z];
@@ -156,39 +194,56 @@ __global__ void __launch_bounds__(256, 2) k_jac_bin(gm_grid g, const double* x_s
         }
       }
     }
-    double xa = 0.0, la = 0.0;
-    if (t < PG * 3 && t / 3 < np) {
-      xa = x_s[(long)(p0 + t / 3) * 3 + t % 3];
-      la = lp_s[(long)(p0 + t / 3) * 3 + t % 3];
+    xa = 0.0;
+    la = 0.0;
+    if (t >= GT0 && t < GT0 + PG * 3 && (t - GT0) / 3 < np) {
+      xa = x_s[(long)(p0 + (t - GT0) / 3) * 3 + (t - GT0) % 3];
+      la = lp_s[(long)(p0 + (t - GT0) / 3) * 3 + (t - GT0) % 3];
     }
-    __syncthreads();  // the previous group's LDS reads are done
-    if (t < PG * 3) {
-      const int p = t / 3, a = t % 3;
+  };
+  // shape-function tables of the group whose positions are in xa / la (np particles)
+  auto axis_tables = [&](int buf, int np) {
+    if (t >= GT0 && t < GT0 + PG * 3) {
+      const int p = (t - GT0) / 3, a = (t - GT0) % 3;
       if (p < np) {
         const Axis ax = gimp_axis(xa, la, g.h[a], g.inv_h[a]);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          AXT[p][a][i] = (TC)ax.w[i];
-          AXT[p][a][3 + i] = (TC)ax.g[i];
-          AXT[p][a][6 + i] = (TC)ax.wa[i];
+          AXT_[buf][p][a][i] = (TC)ax.w[i];
+          AXT_[buf][p][a][3 + i] = (TC)ax.g[i];
+          AXT_[buf][p][a][6 + i] = (TC)ax.wa[i];
         }
       } else {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) AXT[p][a][i] = (TC)0;
+        for (int i = 0; i < 9; ++i) AXT_[buf][p][a][i] = (TC)0;
       }
     }
-    __syncthreads();
-    for (int q = t; q < PG * 5 * 32; q += 256) {
-      const int p = q / 160, beta = (q / 32) % 5, J = q % 32;
-      BT[p][beta][J] = (J < 27) ? gcomp<TC>(AXT[p], J, beta) : (TC)0;
+  };
+  int p0 = ks, cur = 0;
+  if (PF) {
+    load_group(p0, min(PG, ke - p0));
+    axis_tables(0, min(PG, ke - p0));
+  }
+  while (true) {
+    const int np = min(PG, ke - p0);
+    if (!PF) load_group(p0, np);
+    __syncthreads();  // the previous group's LDS reads are done (and, with PF, this one's tables written)
+    if (!PF) {
+      axis_tables(cur, np);
+      __syncthreads();
     }
-    // E[p][I][r][z] = (L_I D_p)[r][z]: u rows sum_b dw/dx_b D[(b,r)] + w D[T_r];
+    const TC(*AXT)[3][9] = AXT_[cur];
+    for (int q = t; q < PG * 5 * BB; q += 256) {
+      const int p = q / (5 * BB), beta = (q / BB) % 5, J = q % BB;
+      BT(p, beta, J) = (J < 27) ? gcomp<TC>(AXT[p], J, beta) : (TC)0;
+    }
+    // E_p[I][z][r] = (L_I D_p)[r][z]: u rows sum_b dw/dx_b D[(b,r)] + w D[T_r];
     // the pressure row w D[Q] + sum_b dw/dx_b D[M_b] + (w - w_avg) D[S]
 #pragma unroll
     for (int j = 0; j < NQ; ++j) {
       const int q = t + 256 * j;
-      if (q >= PG * 4 * NZ) break;
-      const int p = q / (4 * NZ), rz = q % (4 * NZ), r = rz / NZ, z = rz % NZ;
+      if (q >= PG * NI) break;
+      const int p = q / NI, zr = q % NI, z = zr >> 2, r = zr & 3;
       const TC d0 = dv[j][0], d1 = dv[j][1], d2 = dv[j][2], dw = dv[j][3], dwa = dv[j][4];
       // sum-factorised over the tensor-product axes: with u = gx d0 + wx dw, v = wx d1,
       // q = wx d2, a = wax dwa per x node, E = wz (wy u + gy v) + gz (wy q) + waz (way a)
@@ -212,57 +267,78 @@ __global__ void __launch_bounds__(256, 2) k_jac_bin(gm_grid g, const double* x_s
         for (int iy = 0; iy < 3; ++iy) {
           const TC U = wy[iy] * u + gy[iy] * v, Q = wy[iy] * qx, A2 = ay[iy] * a;
 #pragma unroll
-          for (int iz = 0; iz < 3; ++iz) E(p, ix + 3 * iy + 9 * iz, r, z) = wz[iz] * U + gz[iz] * Q + az[iz] * A2;
+          for (int iz = 0; iz < 3; ++iz) E(p, ix + 3 * iy + 9 * iz, z, r) = wz[iz] * U + gz[iz] * Q + az[iz] * A2;
         }
       }
     }
-    __syncthreads();
-#pragma unroll 1
-    for (int kst = 0; kst < KST; ++kst) {
-      // B operand (k = (p, beta), column J) of both column tiles
-      const int k = kst * 4 + (lane >> 4), p = k / 5, beta = k % 5;
-      // particles past np have all-zero axis tables, hence zero B
-      const TC b0 = BT[p][beta][lane & 15];
-      const TC b1 = BT[p][beta][16 + (lane & 15)];
-      int zc[4];
+    // the next group: the rest of this bin, else the first of the next non-empty bin
+    int nb = b, nks = ks, nke = ke, np0 = p0 + PG;
+    const bool bin_done = np0 >= ke;
+    if (bin_done) {
+      for (nb = b + gridDim.x; nb < n_bins; nb += gridDim.x) {
+        nks = cell_start[nb];
+        nke = cell_start[nb + 1];
+        if (nke > nks) break;
+      }
+      np0 = nks;
+    }
+    const bool more = nb < n_bins;
+    if (PF && more) load_group(np0, min(PG, nke - np0));
+    __syncthreads();  // E and B tables complete
+    // particles past np have all-zero axis tables, hence zero B
 #pragma unroll
-      for (int c = 0; c < 4; ++c) zc[c] = zsel(c, beta);
+    for (int kst = 0; kst < KST; ++kst) {
+      const int beta = kst % 5, pb = (kst / 5) * 4;  // constants once unrolled
+      const TC bk = BT_[bbase + pb * BP + beta * BB];
+      // the step's A operands are read before its products are issued
       TC av[4][4];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int tt = wave + 4 * s, mt = tt >> 1;
-        const int rt = lane & 15;
-        const int I = min(mt * 4 + (Mfma<TC>::r_major ? (rt & 3) : (rt >> 2)), 27);
-        const int r = Mfma<TC>::r_major ? (rt >> 2) : (rt & 3);
+      for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) av[s][c] = E(p, I, r, zc[c]);
-      }
+        for (int c = 0; c < 4; ++c) {
+          const int z = zsel(c, beta);
+          av[s][c] = (z == ZERO) ? (TC)0 : E_[abase[s] + pb * EP + z * 4];  // z resolved at compile time
+        }
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const TC bk = ((wave + 4 * s) & 1) ? b1 : b0;
+        if (s == 3 && wave >= 2) break;  // wave-uniform: no fourth tile
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[s][c] = Mfma<TC>::step(av[s][c], bk, acc[s][c]);
+        for (int c = 0; c < 4; ++c)
+          if (zsel(c, beta) != ZERO) acc[s][c] = Mfma<TC>::step(av[s][c], bk, acc[s][c]);
       }
     }
-  }
-  // lane holds rows r = 0..3 of node I_loc = 4 mt + lane / 16 in column J_loc =
-  // 16 nt + lane % 16: one whole 4 x 4 block
-  TC* out = cb + (long)b * CB_BIN;
+    if (bin_done) {
+      // lane holds rows r = 0..3 of node I_loc = 4 mt + lane / 16 in column J_loc =
+      // 16 nt + lane % 16: one whole 4 x 4 block
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int tt = wave + 4 * s, mt = tt >> 1, nt = tt & 1;
-    const int I = mt * 4 + (lane >> 4), J = nt * 16 + (lane & 15);
-    if (I < 27 && J < 27) {
-      TC* o = out + (long)(I * 27 + J) * 16;
+      for (int s = 0; s < 4; ++s) {
+        const int mt = (wave >> 1) + 2 * s, nt = wave & 1;
+        const int I = mt * 4 + (lane >> 4), J = nt * 16 + (lane & 15);
+        if (I < 27 && J < 27) {
+          TC* o = cb + chunk_at(I, b) + J * 16;
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+          for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) o[r * 4 + c] = acc[s][c][r];
+            for (int c = 0; c < 4; ++c) o[r * 4 + c] = acc[s][c][r];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[s][c] = ACC{0, 0, 0, 0};
+      }
     }
+    if (!more) break;
+    if (PF) {
+      cur ^= 1;
+      axis_tables(cur, min(PG, nke - np0));  // read after the next barrier
+    }
+    b = nb;
+    ks = nks;
+    ke = nke;
+    p0 = np0;
   }
 }
 
 #undef E
+#undef BT
 
 // One workgroup per block row I, four lanes per block (one per block row r).  The
 // row's 27 candidate bins (b = I - o, -1 when outside the grid or empty) are read
@@ -307,7 +383,7 @@ __global__ void __launch_bounds__(512) k_jac_gather(gm_grid g, const int* cell_s
     const int kx = d[0] + o % 3, ky = d[1] + (o / 3) % 3, kz = d[2] + o / 9;
     const int bn = BIN[o];
     if (bn >= 0 && kx >= 0 && kx < 3 && ky >= 0 && ky < 3 && kz >= 0 && kz < 3) {
-      const TC* src = cb + (long)bn * CB_BIN + (long)(o * 27 + (kx + 3 * ky + 9 * kz)) * 16 + r * 4;
+      const TC* src = cb + chunk_at(o, bn) + (kx + 3 * ky + 9 * kz) * 16 + r * 4;
 #pragma unroll
       for (int c = 0; c < 4; ++c) s[c] += (double)src[c];
     }
@@ -322,6 +398,117 @@ __global__ void __launch_bounds__(512) k_jac_gather(gm_grid g, const int* cell_s
     double v = (fr && fc) ? -s[c] : 0.0;
     if (diag && r == c && !fr) v = 1.0;
     out[c] = (TV)v;
+  }
+}
+
+// The same sums with coalesced reads: one workgroup per block row I stages the row's
+// <= 27 bin chunks (bin I - o, I_loc = o: 27 blocks, contiguous) in LDS
+// with whole 16-byte vector loads, OB chunks at a time, and every (block, row r) item
+// then adds its <= 27 contributions from LDS in the same order (o = 0..26) as
+// k_jac_gather: both write the same bits.
+template <typename TC, typename TV, int OB>
+__global__ void __launch_bounds__(256, 3) k_jac_gather_staged(gm_grid g, const int* cell_start, const TC* cb,
+                                                           const uint8_t* freem, const int* row_ptr,
+                                                           const uint64_t* row_mask, TV* vals) {
+  static_assert(27 % OB == 0, "chunk rounds cover o = 0..26");
+  constexpr int CH = 27 * 16;          // values per chunk
+  constexpr int CV = CH * sizeof(TC) / 16;  // 16-byte vectors per chunk
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  __shared__ int SLOT_BIT[125];
+  __shared__ int BIN[27];
+  __shared__ u4 ST[OB][CV];
+  const int I = blockIdx.x, t = threadIdx.x;
+  const uint64_t m0 = row_mask[2 * I], m1 = row_mask[2 * I + 1];
+  const int nx1 = g.nx + 1, nxy = (g.nx + 1) * (g.ny + 1);
+  const int ci[3] = {I % nx1, (I / nx1) % (g.ny + 1), I / nxy};
+  if (t < 125) {
+    const uint64_t w = (t < 64) ? m0 : m1;
+    const int sh = t & 63;
+    if ((w >> sh) & 1ull) {
+      const int below = (t < 64) ? __popcll(m0 & ((1ull << sh) - 1ull))
+                                 : __popcll(m0) + __popcll(m1 & ((1ull << sh) - 1ull));
+      SLOT_BIT[below] = t;
+    }
+  } else if (t >= 128 && t < 128 + 27) {
+    const int o = t - 128, ox = o % 3, oy = (o / 3) % 3, oz = o / 9;
+    const int bc[3] = {ci[0] - ox, ci[1] - oy, ci[2] - oz};
+    int bn = -1;
+    if (bc[0] >= 0 && bc[0] <= g.nx && bc[1] >= 0 && bc[1] <= g.ny && bc[2] >= 0 && bc[2] <= g.nz) {
+      bn = bc[0] + bc[1] * nx1 + bc[2] * nxy;
+      if (cell_start[bn + 1] == cell_start[bn]) bn = -1;
+    }
+    BIN[o] = bn;
+  }
+  __syncthreads();
+  const int nnz = __popcll(m0) + __popcll(m1);
+  // items (slot, r) = (it / 4, it % 4), it = t and t + 256
+  int dd[2][3];
+  bool live[2];
+  double s[2][4];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int it = t + 256 * h;
+    live[h] = it < 4 * nnz;
+    const int bit = live[h] ? SLOT_BIT[it >> 2] : 62;
+    dd[h][0] = bit % 5 - 2;
+    dd[h][1] = (bit / 5) % 5 - 2;
+    dd[h][2] = bit / 25 - 2;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s[h][c] = 0.0;
+  }
+#pragma unroll 1
+  for (int o0 = 0; o0 < 27; o0 += OB) {
+    if (o0) __syncthreads();  // the previous round's LDS reads are done
+    // all of a thread's loads are issued before its first LDS store
+    constexpr int NJ = (OB * CV + 255) / 256;
+    u4 tmp[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int q = t + 256 * j;
+      tmp[j] = u4{0u, 0u, 0u, 0u};
+      if (q < OB * CV) {
+        const int oo = q / CV, f = q % CV;
+        const int bn = BIN[o0 + oo];
+        if (bn >= 0) tmp[j] = reinterpret_cast<const u4*>(cb + chunk_at(o0 + oo, bn))[f];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int q = t + 256 * j;
+      if (q < OB * CV) ST[q / CV][q % CV] = tmp[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (!live[h]) continue;
+      const int r = (t + 256 * h) & 3;
+#pragma unroll 3
+      for (int oo = 0; oo < OB; ++oo) {
+        const int o = o0 + oo;
+        const int kx = dd[h][0] + o % 3, ky = dd[h][1] + (o / 3) % 3, kz = dd[h][2] + o / 9;
+        if (BIN[o] >= 0 && kx >= 0 && kx < 3 && ky >= 0 && ky < 3 && kz >= 0 && kz < 3) {
+          const TC* src = reinterpret_cast<const TC*>(&ST[oo][0]) + (kx + 3 * ky + 9 * kz) * 16 + r * 4;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) s[h][c] += (double)src[c];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (!live[h]) continue;
+    const int it = t + 256 * h, slot = it >> 2, r = it & 3;
+    const int Jn = I + dd[h][0] + dd[h][1] * nx1 + dd[h][2] * nxy;
+    const bool diag = (Jn == I);
+    const bool fr = freem[I * 4 + r];
+    TV* out = vals + (long)(row_ptr[I] + slot) * 16 + r * 4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const bool fc = freem[Jn * 4 + c];
+      double v = (fr && fc) ? -s[h][c] : 0.0;
+      if (diag && r == c && !fr) v = 1.0;
+      out[c] = (TV)v;
+    }
   }
 }
 
@@ -529,33 +716,80 @@ extern "C" size_t gm_node_jacobian_work_bytes(const gm_grid* g, int tangent_fp32
   return (size_t)g->n_nodes * (size_t)CB_BIN * (tangent_fp32 ? sizeof(float) : sizeof(double));
 }
 
-extern "C" int gm_jacobian_bins(const gm_grid* g, const gm_bins* b, const void* D, int tangent_fp32, void* work,
-                                size_t work_bytes, hipStream_t s) {
-  GM_REQUIRE(g && b && D && work, "gm_jacobian_bins: null argument");
-  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_jacobian_bins: the 3D u-p form only");
-  GM_REQUIRE(work_bytes >= gm_node_jacobian_work_bytes(g, tangent_fp32),
-             "gm_jacobian_bins: workspace smaller than gm_node_jacobian_work_bytes");
+// ------------------------------------------------------------ variants and their tuner
+// Per-bin kernel variants (particles per group, workgroups per CU, grid):
+//   FP32 tangent: 0 (8, 2, one per bin), 1 (8, 2, persistent), 2 (4, 4, persistent), 3 (4, 3, persistent)
+//   FP64 tangent: 0 (4, 2, one per bin), 1 (4, 2, persistent), 2 (8, 1, one per bin), 3 (8, 1, persistent)
+// "persistent": one workgroup per resident slot (CUs x workgroups per CU), each
+// sweeping bins blockIdx.x + k gridDim.x, with the next group's loads issued before
+// the current group's products; "one per bin": loads at the top of each group.  Gather
+// variants: 0 (per-lane reads from the workspace), 1 (all 27 chunks of a row staged in
+// LDS at once; 9 at a time in FP64), 2 (9 at a time; 3 in FP64).  Every variant writes
+// the same bits.
+namespace {
+constexpr int N_BIN_VARIANTS = 4, N_GATHER_VARIANTS = 3;
+int g_bin_variant[2] = {0, 0};     // [tangent_fp32]
+int g_gather_variant[2] = {0, 0};  // [tangent_fp32]
+
+int cu_count() {
+  static int n = 0;
+  if (n == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return 0;
+    n = pr.multiProcessorCount;
+  }
+  return n;
+}
+
+int launch_bins(const gm_grid* g, const gm_bins* b, const void* D, int fp32, int variant, void* work,
+                hipStream_t s) {
   const int n = g->n_nodes;
-  if (n == 0) return 0;
-  if (tangent_fp32)
-    k_jac_bin<float, float><<<n, 256, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, (const float*)D, (float*)work);
-  else
-    k_jac_bin<double, double><<<n, 256, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, (const double*)D,
-                                                (double*)work);
+  const int cus = cu_count();
+  GM_REQUIRE(cus > 0, "gm_jacobian_bins: no device properties");
+#define GM_BIN(TT, PG, OCC, PERSIST)                                                                     \
+  if (PERSIST)                                                                                           \
+    k_jac_bin<TT, TT, PG, OCC, true><<<std::min(n, cus * OCC), 256, 0, s>>>(*g, b->x_s, b->lp_s,          \
+                                                                        b->cell_start, n, (const TT*)D, \
+                                                                        (TT*)work);                     \
+  else                                                                                                   \
+    k_jac_bin<TT, TT, PG, OCC, false><<<n, 256, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, n, (const TT*)D, \
+                                                      (TT*)work)
+  if (fp32) {
+    switch (variant) {
+      case 0: GM_BIN(float, 8, 2, false); break;
+      case 1: GM_BIN(float, 8, 2, true); break;
+      case 2: GM_BIN(float, 4, 4, true); break;
+      default: GM_BIN(float, 4, 3, true); break;
+    }
+  } else {
+    switch (variant) {
+      case 0: GM_BIN(double, 4, 2, false); break;
+      case 1: GM_BIN(double, 4, 2, true); break;
+      case 2: GM_BIN(double, 8, 1, false); break;
+      default: GM_BIN(double, 8, 1, true); break;
+    }
+  }
+#undef GM_BIN
   GM_LAUNCH_CHECK();
   return 0;
 }
 
-extern "C" int gm_jacobian_gather(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, int tangent_fp32,
-                                  const void* work, const uint8_t* freem, void* vals, int vals_fp32, hipStream_t s) {
-  GM_REQUIRE(g && b && pat && work && freem && vals, "gm_jacobian_gather: null argument");
-  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_jacobian_gather: the 3D u-p form only");
+int launch_gather(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, int fp32, int variant,
+                  const void* work, const uint8_t* freem, void* vals, int vals_fp32, hipStream_t s) {
   const int n = g->n_nodes;
-  if (n == 0) return 0;
-#define GM_GATHER(TCC, TVV)                                                                              \
-  k_jac_gather<TCC, TVV><<<n, 512, 0, s>>>(*g, b->cell_start, (const TCC*)work, freem, pat->row_ptr, \
-                                           pat->row_mask, (TVV*)vals)
-  if (tangent_fp32) {
+#define GM_GATHER(TCC, TVV)                                                                                      \
+  if (variant == 0)                                                                                              \
+    k_jac_gather<TCC, TVV><<<n, 512, 0, s>>>(*g, b->cell_start, (const TCC*)work, freem, pat->row_ptr,         \
+                                             pat->row_mask, (TVV*)vals);                                       \
+  else if (variant == 1)                                                                                         \
+    k_jac_gather_staged<TCC, TVV, sizeof(TCC) == 4 ? 27 : 9><<<n, 256, 0, s>>>(                                 \
+        *g, b->cell_start, (const TCC*)work, freem, pat->row_ptr, pat->row_mask, (TVV*)vals);                  \
+  else                                                                                                           \
+    k_jac_gather_staged<TCC, TVV, sizeof(TCC) == 4 ? 9 : 3><<<n, 256, 0, s>>>(                                  \
+        *g, b->cell_start, (const TCC*)work, freem, pat->row_ptr, pat->row_mask, (TVV*)vals)
+  if (fp32) {
     if (vals_fp32) GM_GATHER(float, float);
     else GM_GATHER(float, double);
   } else {
@@ -564,6 +798,97 @@ extern "C" int gm_jacobian_gather(const gm_grid* g, const gm_bins* b, const gm_p
   }
 #undef GM_GATHER
   GM_LAUNCH_CHECK();
+  return 0;
+}
+}  // namespace
+
+extern "C" int gm_jacobian_bins_variant(const gm_grid* g, const gm_bins* b, const void* D, int tangent_fp32,
+                                        int variant, void* work, size_t work_bytes, hipStream_t s) {
+  GM_REQUIRE(g && b && D && work, "gm_jacobian_bins: null argument");
+  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_jacobian_bins: the 3D u-p form only");
+  GM_REQUIRE(variant >= 0 && variant < N_BIN_VARIANTS, "gm_jacobian_bins: variant out of range");
+  GM_REQUIRE(work_bytes >= gm_node_jacobian_work_bytes(g, tangent_fp32),
+             "gm_jacobian_bins: workspace smaller than gm_node_jacobian_work_bytes");
+  if (g->n_nodes == 0) return 0;
+  return launch_bins(g, b, D, tangent_fp32 ? 1 : 0, variant, work, s);
+}
+
+extern "C" int gm_jacobian_bins(const gm_grid* g, const gm_bins* b, const void* D, int tangent_fp32, void* work,
+                                size_t work_bytes, hipStream_t s) {
+  return gm_jacobian_bins_variant(g, b, D, tangent_fp32, g_bin_variant[tangent_fp32 ? 1 : 0], work, work_bytes, s);
+}
+
+extern "C" int gm_jacobian_gather_variant(const gm_grid* g, const gm_bins* b, const gm_pattern* pat,
+                                          int tangent_fp32, int variant, const void* work, const uint8_t* freem,
+                                          void* vals, int vals_fp32, hipStream_t s) {
+  GM_REQUIRE(g && b && pat && work && freem && vals, "gm_jacobian_gather: null argument");
+  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_jacobian_gather: the 3D u-p form only");
+  GM_REQUIRE(variant >= 0 && variant < N_GATHER_VARIANTS, "gm_jacobian_gather: variant out of range");
+  if (g->n_nodes == 0) return 0;
+  return launch_gather(g, b, pat, tangent_fp32 ? 1 : 0, variant, work, freem, vals, vals_fp32, s);
+}
+
+extern "C" int gm_jacobian_gather(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, int tangent_fp32,
+                                  const void* work, const uint8_t* freem, void* vals, int vals_fp32, hipStream_t s) {
+  return gm_jacobian_gather_variant(g, b, pat, tangent_fp32, g_gather_variant[tangent_fp32 ? 1 : 0], work, freem,
+                                    vals, vals_fp32, s);
+}
+
+// Times every per-bin and gather variant on the given tangent (`reps` launches each
+// after one warm-up, HIP events on the stream), keeps the fastest of each for later
+// assemblies in this precision and returns them with the average times (ms[0..3]
+// per-bin variants, ms[4..6] gather variants).  The matrix and the workspace are
+// overwritten with the assembly.
+extern "C" int gm_jacobian_autotune(const gm_grid* g, const gm_bins* b, const gm_pattern* pat, const void* D,
+                                    int tangent_fp32, const uint8_t* freem, void* vals, int vals_fp32, void* work,
+                                    size_t work_bytes, int reps, int* best_bins, int* best_gather, float* ms,
+                                    hipStream_t s) {
+  GM_REQUIRE(g && b && pat && D && freem && vals && work && best_bins && best_gather && ms && reps >= 1,
+             "gm_jacobian_autotune: bad argument");
+  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_jacobian_autotune: the 3D u-p form only");
+  GM_REQUIRE(work_bytes >= gm_node_jacobian_work_bytes(g, tangent_fp32),
+             "gm_jacobian_autotune: workspace smaller than gm_node_jacobian_work_bytes");
+  const int fp = tangent_fp32 ? 1 : 0;
+  if (g->n_nodes == 0) {
+    *best_bins = g_bin_variant[fp];
+    *best_gather = g_gather_variant[fp];
+    return 0;
+  }
+  hipEvent_t e0, e1;
+  GM_HIP(hipEventCreate(&e0));
+  GM_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  auto time = [&](auto&& launch, float* out) -> int {
+    int r = launch();
+    if (r) return r;
+    GM_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < reps; ++k)
+      if ((r = launch())) return r;
+    GM_HIP(hipEventRecord(e1, s));
+    GM_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    GM_HIP(hipEventElapsedTime(&t, e0, e1));
+    *out = t / reps;
+    return 0;
+  };
+  int bb = 0, bg = 0;
+  for (int v = 0; v < N_BIN_VARIANTS && !rc; ++v) {
+    rc = time([&]() { return launch_bins(g, b, D, fp, v, work, s); }, &ms[v]);
+    if (!rc && ms[v] < ms[bb]) bb = v;
+  }
+  if (!rc) rc = launch_bins(g, b, D, fp, bb, work, s);
+  for (int v = 0; v < N_GATHER_VARIANTS && !rc; ++v) {
+    rc = time([&]() { return launch_gather(g, b, pat, fp, v, work, freem, vals, vals_fp32, s); },
+              &ms[N_BIN_VARIANTS + v]);
+    if (!rc && ms[N_BIN_VARIANTS + v] < ms[N_BIN_VARIANTS + bg]) bg = v;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  g_bin_variant[fp] = bb;
+  g_gather_variant[fp] = bg;
+  *best_bins = bb;
+  *best_gather = bg;
   return 0;
 }
 

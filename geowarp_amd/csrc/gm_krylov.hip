@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -648,6 +649,192 @@ extern "C" int gm_bicgstab(int nb, int fp32, int n_rows, const int* row_ptr, con
   return -1;
 }
 
+// ------------------------------------------------------------------ PCG
+// Conjugate gradients (Hestenes & Stiefel 1952) with the block-Jacobi preconditioner
+// for symmetric positive definite systems (the consistent-mass projection): one SpMV
+// with the fused p.Ap partials and two reductions per iteration, every scalar on the
+// device; the host reads ||r|| every `check_every` iterations.
+enum { C_RZ = 0, C_RZN, C_PQ, C_RR, C_BB, C_ALPHA, C_BETA, C_FLAG, C_NSC };
+
+// x = 0, r = b, z = Dinv b, p = z; per-workgroup partials of r.z and r.r
+template <int NB, typename TM, typename TV>
+__global__ void k_pcg_start(int n_rows, const TV* b, const TM* dinv, TV* x, TV* r, TV* z, TV* p, double* part) {
+  __shared__ double a[RED_T], c[RED_T];
+  double s0 = 0.0, s1 = 0.0;
+  for (int row = blockIdx.x * RED_T + threadIdx.x; row < n_rows; row += gridDim.x * RED_T) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      x[row * NB + j] = (TV)0;
+      r[row * NB + j] = b[row * NB + j];
+    }
+    apply_dinv<NB, TM, TV>(dinv, row, r, z);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const double rv = (double)r[row * NB + j], zv = (double)z[row * NB + j];
+      p[row * NB + j] = z[row * NB + j];
+      s0 += rv * zv;
+      s1 += rv * rv;
+    }
+  }
+  a[threadIdx.x] = s0;
+  c[threadIdx.x] = s1;
+  __syncthreads();
+  for (int o = RED_T / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      a[threadIdx.x] += a[threadIdx.x + o];
+      c[threadIdx.x] += c[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a[0];
+    part[2 * blockIdx.x + 1] = c[0];
+  }
+}
+
+__global__ void k_pcg_init_sc(double* sc) {
+  sc[C_BB] = sc[C_RR];
+  sc[C_FLAG] = 0.0;
+}
+
+// alpha = (r.z) / (p.Ap); a non-positive or non-finite p.Ap stops the iteration (flag)
+__global__ void k_pcg_alpha(double* sc) {
+  const double pq = sc[C_PQ];
+  const bool ok = pq > 0.0 && pq < 1e308 && sc[C_FLAG] == 0.0;
+  sc[C_ALPHA] = ok ? sc[C_RZ] / pq : 0.0;
+  if (!ok) sc[C_FLAG] = 1.0;
+}
+
+// x += alpha p, r -= alpha q, z = Dinv r; per-workgroup partials of r.z and r.r
+template <int NB, typename TM, typename TV>
+__global__ void k_pcg_xr(int n_rows, const double* sc, TV* x, TV* r, const TV* p, const TV* q, const TM* dinv, TV* z,
+                         double* part) {
+  __shared__ double a[RED_T], c[RED_T];
+  const double alpha = sc[C_ALPHA];
+  double s0 = 0.0, s1 = 0.0;
+  for (int row = blockIdx.x * RED_T + threadIdx.x; row < n_rows; row += gridDim.x * RED_T) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int i = row * NB + j;
+      x[i] = (TV)((double)x[i] + alpha * (double)p[i]);
+      r[i] = (TV)((double)r[i] - alpha * (double)q[i]);
+    }
+    apply_dinv<NB, TM, TV>(dinv, row, r, z);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const double rv = (double)r[row * NB + j];
+      s0 += rv * (double)z[row * NB + j];
+      s1 += rv * rv;
+    }
+  }
+  a[threadIdx.x] = s0;
+  c[threadIdx.x] = s1;
+  __syncthreads();
+  for (int o = RED_T / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      a[threadIdx.x] += a[threadIdx.x + o];
+      c[threadIdx.x] += c[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a[0];
+    part[2 * blockIdx.x + 1] = c[0];
+  }
+}
+
+// beta = (r.z)_new / (r.z); keeps (r.z)_new for the next step
+__global__ void k_pcg_beta(double* sc) {
+  const double rz = sc[C_RZ];
+  sc[C_BETA] = (rz != 0.0) ? sc[C_RZN] / rz : 0.0;
+  sc[C_RZ] = sc[C_RZN];
+}
+
+template <typename TV>
+__global__ void k_pcg_p(long n, const double* sc, const TV* z, TV* p) {
+  const double beta = sc[C_BETA];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = (TV)((double)z[i] + beta * (double)p[i]);
+}
+
+template <int NB, typename TM, typename TV>
+static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, const TV* b, TV* x, gm_krylov_ws* ws,
+                    double tol, int maxiter, int check_every, int* iters_out, double* relres_out, hipStream_t st) {
+  long n = (long)n_rows * NB;
+  TV *r = (TV*)ws->r, *p = (TV*)ws->p, *q = (TV*)ws->v, *z = (TV*)ws->z;
+  TM* dinv = (TM*)ws->dinv;
+  double* sc = ws->sc;
+  double* part = ws->part;
+  const int rb = gm_blocks(n_rows, 256);
+  const int db = std::min(1024, gm_blocks(n_rows, RED_T));
+  const int sb = spmv_blocks(n_rows, NB);
+  k_diag_inv<NB, TM><<<rb, 256, 0, st>>>(n_rows, rp, ci, vals, dinv);
+  k_pcg_start<NB, TM, TV><<<db, RED_T, 0, st>>>(n_rows, b, dinv, x, r, z, p, part);
+  k_sum_parts<<<1, RED_T, 0, st>>>(part, db, sc + C_RZ, sc + C_RR);
+  k_pcg_init_sc<<<1, 1, 0, st>>>(sc);
+  GM_LAUNCH_CHECK();
+  double hs[C_NSC];
+  GM_HIP(hipMemcpyAsync(hs, sc, sizeof(double) * C_NSC, hipMemcpyDeviceToHost, st));
+  GM_HIP(hipStreamSynchronize(st));
+  const double bnorm = sqrt(hs[C_BB]);
+  *iters_out = 0;
+  *relres_out = 0.0;
+  if (!(bnorm == bnorm) || bnorm > 1e300) {
+    gm::set_error("pcg: right-hand side is not finite");
+    return -5;
+  }
+  if (bnorm == 0.0) return 0;
+  int it = 0;
+  double rel = 1.0;
+  while (it < maxiter) {
+    const int nrun = std::min(check_every, maxiter - it);
+    for (int k = 0; k < nrun; ++k) {
+      launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, p, q, p, nullptr, part, st);
+      k_sum_parts<<<1, RED_T, 0, st>>>(part, sb, sc + C_PQ, nullptr);
+      k_pcg_alpha<<<1, 1, 0, st>>>(sc);
+      k_pcg_xr<NB, TM, TV><<<db, RED_T, 0, st>>>(n_rows, sc, x, r, p, q, dinv, z, part);
+      k_sum_parts<<<1, RED_T, 0, st>>>(part, db, sc + C_RZN, sc + C_RR);
+      k_pcg_beta<<<1, 1, 0, st>>>(sc);
+      k_pcg_p<TV><<<gm_blocks(n, 256), 256, 0, st>>>(n, sc, z, p);
+    }
+    GM_LAUNCH_CHECK();
+    it += nrun;
+    GM_HIP(hipMemcpyAsync(hs, sc, sizeof(double) * C_NSC, hipMemcpyDeviceToHost, st));
+    GM_HIP(hipStreamSynchronize(st));
+    rel = sqrt(fmax(hs[C_RR], 0.0)) / bnorm;
+    if (!(hs[C_RR] == hs[C_RR])) {
+      gm::set_error("pcg: residual became NaN after %d iterations", it);
+      *iters_out = it;
+      *relres_out = hs[C_RR];
+      return -5;
+    }
+    if (rel <= tol || hs[C_FLAG] != 0.0) break;  // converged, or p.Ap <= 0: the caller sees rel
+  }
+  *iters_out = it;
+  *relres_out = rel;
+  return 0;
+}
+
+extern "C" int gm_pcg(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
+                      const void* b, void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int* iters,
+                      double* relres, hipStream_t s) {
+  GM_REQUIRE(ws && iters && relres, "gm_pcg: null argument");
+  if (check_every < 1) check_every = 1;
+#define GM_PCG(NB)                                                                                              \
+  if (nb == NB)                                                                                                 \
+    return fp32 ? pcg_impl<NB, float, float>(n_rows, row_ptr, col_idx, (const float*)vals, (const float*)b,     \
+                                             (float*)x, ws, tol, maxiter, check_every, iters, relres, s)        \
+                : pcg_impl<NB, double, double>(n_rows, row_ptr, col_idx, (const double*)vals, (const double*)b, \
+                                               (double*)x, ws, tol, maxiter, check_every, iters, relres, s);
+  GM_PCG(1)
+  GM_PCG(2)
+  GM_PCG(3)
+  GM_PCG(4)
+#undef GM_PCG
+  gm::set_error("gm_pcg: unsupported block size %d", nb);
+  return -1;
+}
+
 // y = A x  (A in FP64 or FP32, x and y in FP64 or FP32; FP64 accumulation)
 extern "C" int gm_bsr_spmv(int nb, int mat_fp32, int vec_fp32, int n_rows, const int* row_ptr, const int* col_idx,
                            const void* vals, const void* x, void* y, hipStream_t s) {
@@ -778,8 +965,12 @@ extern "C" int gm_fix_empty_diag(int n_rows, const int* row_ptr, const int* col_
 // x <- x + M^{-1} V y at the end of every cycle; restart from r = b - A x.
 
 // partial dots of w with basis vectors V[0..k): part[i*nchunk + c]
+// gate: when given, the launch does nothing unless *gate != 0 (a decision taken on the
+// device inside an enqueued Arnoldi step)
 template <typename TV>
-__global__ void k_multidot(long n, int k, const TV* V, long ldv, const TV* w, int nchunk, double* part) {
+__global__ void k_multidot(const double* gate, long n, int k, const TV* V, long ldv, const TV* w, int nchunk,
+                           double* part) {
+  if (gate && *gate == 0.0) return;
   __shared__ double sh[RED_T];
   int i = blockIdx.y;
   double s = 0.0;
@@ -796,7 +987,8 @@ __global__ void k_multidot(long n, int k, const TV* V, long ldv, const TV* w, in
 }
 
 // h[i] = sum_c part[i*nchunk + c]  (one block per basis vector, fixed order)
-__global__ void k_multidot_fin(int nchunk, const double* part, double* h) {
+__global__ void k_multidot_fin(const double* gate, int nchunk, const double* part, double* h) {
+  if (gate && *gate == 0.0) return;
   __shared__ double sh[RED_T];
   int i = blockIdx.x;
   double s = 0.0;
@@ -813,7 +1005,9 @@ __global__ void k_multidot_fin(int nchunk, const double* part, double* h) {
 // w -= sum_i V_i h_i   (h on the device); one row per thread, four independent partial
 // sums over the basis vectors so that four loads are in flight per thread
 template <typename TV>
-__global__ void __launch_bounds__(256) k_multiaxpy(long n, int k, const TV* V, long ldv, const double* h, TV* w) {
+__global__ void __launch_bounds__(256) k_multiaxpy(const double* gate, long n, int k, const TV* V, long ldv,
+                                                   const double* h, TV* w) {
+  if (gate && *gate == 0.0) return;
   long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (r >= n) return;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -843,6 +1037,21 @@ __global__ void __launch_bounds__(256) k_combine(long n, int k, const TV* V, lon
   }
   for (; i < k; ++i) a0 += y[i] * (double)V[(long)i * ldv + r];
   out[r] = (TV)((a0 + a1) + (a2 + a3));
+}
+
+// selective reorthogonalisation test (Daniel, Gragg, Kaufman & Stewart 1976) on the
+// device: gate = 1 when the first pass removed more than half of ||w||^2 (or a norm is NaN)
+__global__ void k_reorth_gate(const double* nrm_after, const double* nrm_before, double* gate) {
+  *gate = (*nrm_after >= 0.5 * *nrm_before) ? 0.0 : 1.0;
+}
+
+// b = a / sqrt(*nrm2) (zero when the norm is not positive: a breakdown the host sees)
+template <typename TV>
+__global__ void k_scale_norm(long n, const TV* a, const double* nrm2, TV* b) {
+  const double q = *nrm2;
+  const double s = (q > 0.0) ? 1.0 / sqrt(q) : 0.0;
+  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x)
+    b[r] = (TV)((double)a[r] * s);
 }
 
 template <typename TV>
@@ -886,11 +1095,138 @@ __global__ void k_add_into(long n, const TV* a, TV* x) {
     x[i] = (TV)((double)x[i] + (double)a[i]);
 }
 
+// ------------------------------------------------------------------ Arnoldi step graphs
+// One Arnoldi step (preconditioner, operator, both Gram-Schmidt passes, normalisation,
+// copy of the Hessenberg column to pinned host memory) is a fixed sequence of launches
+// for a given step index j.  It is captured once per j as a HIP graph and replayed: one
+// launch per step instead of one per kernel, which matters because a multigrid cycle is
+// dozens of short launches.  A graph keeps the pointers and sizes it was captured with,
+// so the cache is keyed on all of them (the preconditioner's included); a solve with
+// another key takes another cache slot (four, least recently used dropped).  The eager
+// path enqueues the same sequence, so both give the same bits.
+template <typename T>
+static void put_key(std::vector<char>& k, const T& v) {
+  const char* p = reinterpret_cast<const char*>(&v);
+  k.insert(k.end(), p, p + sizeof(T));
+}
+
+namespace {
+int g_graphs_on = -1;  // -1: taken from GM_KRYLOV_GRAPHS ("0" turns the graphs off), default on
+bool graphs_on() {
+  if (g_graphs_on < 0) {
+    const char* e = getenv("GM_KRYLOV_GRAPHS");
+    g_graphs_on = (e && e[0] == '0') ? 0 : 1;
+  }
+  return g_graphs_on == 1;
+}
+struct StepGraphs {
+  std::vector<char> key;  // empty: free slot
+  std::vector<hipGraphExec_t> exec;
+  unsigned long last_use = 0;
+};
+constexpr int N_GRAPH_SLOTS = 4;
+StepGraphs g_sg[N_GRAPH_SLOTS];
+unsigned long g_sg_clock = 0;
+hipStream_t g_cap_stream = nullptr;
+double* g_pinned = nullptr;
+size_t g_pinned_n = 0;
+
+void drop_graphs(StepGraphs& e) {
+  for (auto x : e.exec)
+    if (x) (void)hipGraphExecDestroy(x);
+  e.exec.clear();
+  e.key.clear();
+}
+
+// pinned host buffer of at least n doubles for the Hessenberg columns
+int pinned_buffer(size_t n, double** out) {
+  if (g_pinned_n < n) {
+    for (auto& e : g_sg) drop_graphs(e);  // they copy into the old buffer
+    if (g_pinned) (void)hipHostFree(g_pinned);
+    g_pinned = nullptr;
+    g_pinned_n = 0;
+    const size_t want = std::max<size_t>(n, 1024);
+    GM_HIP(hipHostMalloc((void**)&g_pinned, want * sizeof(double), hipHostMallocDefault));
+    g_pinned_n = want;
+  }
+  *out = g_pinned;
+  return 0;
+}
+
+StepGraphs* graphs_for(const std::vector<char>& key) {
+  for (auto& e : g_sg)
+    if (!e.key.empty() && e.key == key) {
+      e.last_use = ++g_sg_clock;
+      return &e;
+    }
+  StepGraphs* victim = nullptr;
+  for (auto& e : g_sg)
+    if (e.key.empty()) {
+      victim = &e;
+      break;
+    }
+  if (!victim) {
+    victim = &g_sg[0];
+    for (auto& e : g_sg)
+      if (e.last_use < victim->last_use) victim = &e;
+  }
+  drop_graphs(*victim);
+  victim->key = key;
+  victim->last_use = ++g_sg_clock;
+  return victim;
+}
+
+// run the enqueued sequence of step j on st: directly, or from its captured graph
+template <typename F>
+int run_step(StepGraphs* sg, int j, hipStream_t st, F&& enqueue) {
+  if (!sg) return enqueue(st);
+  if ((int)sg->exec.size() <= j) sg->exec.resize(j + 1, nullptr);
+  if (!sg->exec[j]) {
+    if (!g_cap_stream) GM_HIP(hipStreamCreateWithFlags(&g_cap_stream, hipStreamNonBlocking));
+    GM_HIP(hipStreamBeginCapture(g_cap_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(g_cap_stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(g_cap_stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess || !graph) {
+      gm::set_error("Arnoldi step capture: %s", hipGetErrorString(e));
+      return -2;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      gm::set_error("Arnoldi step graph: %s", hipGetErrorString(ei));
+      return -2;
+    }
+    sg->exec[j] = ex;
+  }
+  GM_HIP(hipGraphLaunch(sg->exec[j], st));
+  return 0;
+}
+}  // namespace
+
+// Turns the Arnoldi step graphs on (1) or off (0) for later solves; -1 only reads the
+// setting.  Returns the previous setting.
+extern "C" int gm_krylov_graphs(int on) {
+  const int prev = graphs_on() ? 1 : 0;
+  if (on == 0 || on == 1) g_graphs_on = on;
+  return prev;
+}
+
 // Block-Jacobi right preconditioner: out = Dinv in; x += Dinv in.
 template <int NB, typename TM, typename TV>
 struct JacobiPrec {
   int n_rows;
   const TM* dinv;
+  void key(std::vector<char>& k) const {
+    put_key(k, 1);
+    put_key(k, n_rows);
+    put_key(k, dinv);
+  }
   int apply(const TV* in, TV* out, hipStream_t st) const {
     k_apply_dinv<NB, TM, TV><<<gm_blocks(n_rows, 256), 256, 0, st>>>(n_rows, dinv, in, out);
     GM_LAUNCH_CHECK();
@@ -937,6 +1273,33 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
   }
   if (bnorm == 0.0) return 0;
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hcol(m + 2);
+  const int nsl = 2 * (m + 2), gate = 2 * m + 6;
+  double* host = nullptr;
+  {
+    int prc = pinned_buffer(3 * (m + 2), &host);
+    if (prc) return prc;
+  }
+  StepGraphs* sg = nullptr;
+  if (graphs_on()) {
+    std::vector<char> key;
+    put_key(key, NB);
+    put_key(key, (int)sizeof(TM));
+    put_key(key, (int)sizeof(TV));
+    put_key(key, n_rows);
+    put_key(key, rp);
+    put_key(key, ci);
+    put_key(key, vals);
+    put_key(key, V);
+    put_key(key, Zs);
+    put_key(key, z);
+    put_key(key, w);
+    put_key(key, part);
+    put_key(key, dh);
+    put_key(key, m);
+    put_key(key, host);
+    prec.key(key);
+    sg = graphs_for(key);
+  }
   int total = 0;
   double rel = 1.0;
   bool first = true;
@@ -968,42 +1331,45 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
     g[0] = beta;
     int j = 0;
     for (; j < m && total < maxiter; ++j, ++total) {
-      TV* vj = V + (long)j * ldv;
-      TV* vj1 = V + (long)(j + 1) * ldv;
-      TV* zj = Zs ? Zs + (long)j * ldv : z;
-      int prc = prec.apply(vj, zj, st);
-      if (prc) return prc;
-      launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, zj, w, nullptr, nullptr, nullptr, st);
-      // classical Gram-Schmidt with selective reorthogonalisation (Daniel, Gragg,
-      // Kaufman & Stewart 1976): a second pass only when the first one removed
-      // more than half of ||w||^2.  dh: [0, m+2) pass-1 coefficients, [m+2, 2m+4)
-      // pass-2 coefficients, [2m+4] ||w||^2 after the passes, [2m+5] before them.
-      const int nsl = 2 * (m + 2);
-      k_dot2<TV><<<vb, RED_T, 0, st>>>(n, w, w, nullptr, nullptr, part);
-      k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + nsl + 1, nullptr);
-      int nchunk = chunks(j + 1);
-      k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, st>>>(n, j + 1, V, ldv, w, nchunk, part);
-      k_multidot_fin<<<j + 1, RED_T, 0, st>>>(nchunk, part, dh);
-      k_multiaxpy<TV><<<gm_blocks(n, 256), 256, 0, st>>>(n, j + 1, V, ldv, dh, w);
-      k_dot2<TV><<<vb, RED_T, 0, st>>>(n, w, w, nullptr, nullptr, part);
-      k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + nsl, nullptr);
-      GM_LAUNCH_CHECK();
-      std::vector<double> hh(3 * (m + 2));
-      GM_HIP(hipMemcpyAsync(hh.data(), dh, sizeof(double) * 3 * (m + 2), hipMemcpyDeviceToHost, st));
-      GM_HIP(hipStreamSynchronize(st));
-      for (int i = 0; i <= j; ++i) hcol[i] = hh[i];
-      if (!(hh[nsl] >= 0.5 * hh[nsl + 1])) {  // also taken when a norm is NaN
-        k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, st>>>(n, j + 1, V, ldv, w, nchunk, part);
-        k_multidot_fin<<<j + 1, RED_T, 0, st>>>(nchunk, part, dh + (m + 2));
-        k_multiaxpy<TV><<<gm_blocks(n, 256), 256, 0, st>>>(n, j + 1, V, ldv, dh + (m + 2), w);
-        k_dot2<TV><<<vb, RED_T, 0, st>>>(n, w, w, nullptr, nullptr, part);
-        k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh + nsl, nullptr);
+      // one Arnoldi step on the device: z_j = M^-1 v_j, w = A z_j, classical Gram-Schmidt
+      // with selective reorthogonalisation (Daniel, Gragg, Kaufman & Stewart 1976: a second
+      // pass only when the first removed more than half of ||w||^2, decided on the device),
+      // v_{j+1} = w / ||w||, and the column of H copied to pinned host memory.  dh: [0, m+2)
+      // pass-1 coefficients, [m+2, 2m+4) pass-2 coefficients, [2m+4] ||w||^2 after the
+      // passes, [2m+5] before them, [2m+6] the second-pass flag.
+      auto enqueue = [&, j](hipStream_t q) -> int {
+        TV* vj = V + (long)j * ldv;
+        TV* vj1 = V + (long)(j + 1) * ldv;
+        TV* zj = Zs ? Zs + (long)j * ldv : z;
+        int prc = prec.apply(vj, zj, q);
+        if (prc) return prc;
+        launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, zj, w, nullptr, nullptr, nullptr, q);
+        k_dot2<TV><<<vb, RED_T, 0, q>>>(n, w, w, nullptr, nullptr, part);
+        k_sum_parts<<<1, RED_T, 0, q>>>(part, vb, dh + nsl + 1, nullptr);
+        const int nchunk = chunks(j + 1);
+        k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, q>>>(nullptr, n, j + 1, V, ldv, w, nchunk, part);
+        k_multidot_fin<<<j + 1, RED_T, 0, q>>>(nullptr, nchunk, part, dh);
+        k_multiaxpy<TV><<<gm_blocks(n, 256), 256, 0, q>>>(nullptr, n, j + 1, V, ldv, dh, w);
+        k_dot2<TV><<<vb, RED_T, 0, q>>>(n, w, w, nullptr, nullptr, part);
+        k_sum_parts<<<1, RED_T, 0, q>>>(part, vb, dh + nsl, nullptr);
+        k_reorth_gate<<<1, 1, 0, q>>>(dh + nsl, dh + nsl + 1, dh + gate);
+        k_multidot<TV><<<dim3(nchunk, j + 1), RED_T, 0, q>>>(dh + gate, n, j + 1, V, ldv, w, nchunk, part);
+        k_multidot_fin<<<j + 1, RED_T, 0, q>>>(dh + gate, nchunk, part, dh + (m + 2));
+        k_multiaxpy<TV><<<gm_blocks(n, 256), 256, 0, q>>>(dh + gate, n, j + 1, V, ldv, dh + (m + 2), w);
+        const int nc1 = chunks(1);
+        k_multidot<TV><<<dim3(nc1, 1), RED_T, 0, q>>>(dh + gate, n, 1, w, ldv, w, nc1, part);
+        k_multidot_fin<<<1, RED_T, 0, q>>>(dh + gate, nc1, part, dh + nsl);
+        k_scale_norm<TV><<<vb, RED_T, 0, q>>>(n, w, dh + nsl, vj1);
         GM_LAUNCH_CHECK();
-        GM_HIP(hipMemcpyAsync(hh.data() + (m + 2), dh + (m + 2), sizeof(double) * (m + 3), hipMemcpyDeviceToHost,
-                              st));
-        GM_HIP(hipStreamSynchronize(st));
-        for (int i = 0; i <= j; ++i) hcol[i] += hh[(m + 2) + i];
-      }
+        GM_HIP(hipMemcpyAsync(host, dh, sizeof(double) * 3 * (m + 2), hipMemcpyDeviceToHost, q));
+        return 0;
+      };
+      int prc = run_step(sg, j, st, enqueue);
+      if (prc) return prc;
+      GM_HIP(hipStreamSynchronize(st));
+      const double* hh = host;
+      const bool second = hh[gate] != 0.0;
+      for (int i = 0; i <= j; ++i) hcol[i] = second ? hh[i] + hh[(m + 2) + i] : hh[i];
       double hn = sqrt(fmax(hh[nsl], 0.0));
       if (!(hn == hn) || hn > 1e300 || !(hcol[j] == hcol[j])) {
         gm::set_error("gmres: Arnoldi vector became non-finite after %d iterations (preconditioner overflow)",
@@ -1013,7 +1379,6 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
         return -6;
       }
       hcol[j + 1] = hn;
-      if (hn > 0.0) k_scale_copy<TV><<<vb, RED_T, 0, st>>>(n, w, 1.0 / hn, vj1);
       // apply previous rotations, then a new one
       for (int i = 0; i < j; ++i) {
         double t = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
@@ -1370,6 +1735,37 @@ struct Cycle {
   const gm_mg_level* L;
 };
 
+// everything a captured cycle depends on: its parameters and every level's pointers and
+// sizes (a level's nnzb only sizes host-side work, so it is left out)
+inline void cycle_key(const Cycle& cy, std::vector<char>& k) {
+  put_key(k, cy.dim);
+  put_key(k, cy.n_levels);
+  put_key(k, cy.nu_pre);
+  put_key(k, cy.nu_post);
+  put_key(k, cy.coarse_sweeps);
+  put_key(k, cy.omega);
+  for (int l = 0; l < cy.n_levels; ++l) {
+    const gm_mg_level& v = cy.L[l];
+    put_key(k, v.nx);
+    put_key(k, v.ny);
+    put_key(k, v.nz);
+    put_key(k, v.n_nodes);
+    put_key(k, v.row_ptr);
+    put_key(k, v.col_idx);
+    put_key(k, v.row_mask);
+    put_key(k, v.vals);
+    put_key(k, v.dinv);
+    put_key(k, v.freem);
+    put_key(k, v.x);
+    put_key(k, v.b);
+    put_key(k, v.r);
+    put_key(k, v.dense_inv);
+    put_key(k, v.gamma);
+    put_key(k, v.vals_lo);
+    put_key(k, v.row_scale);
+  }
+}
+
 // A fused level operation (k_level MODE) on level l: from the FP16 copy where the level has
 // one (FP32 hierarchies); level 0 holds the Jacobian in the solve precision, coarser levels FP64
 template <int NB, int MODE, typename T>
@@ -1603,6 +1999,11 @@ template <int DIM, int NB, typename T>
 struct MGPrec {
   Cycle cy;
   long n;
+  void key(std::vector<char>& k) const {
+    put_key(k, 2);
+    cycle_key(cy, k);
+    put_key(k, n);
+  }
   int apply(const T* in, T* out, hipStream_t st) const { return vcycle<DIM, NB, T>(cy, 0, in, out, st); }
   int apply_add(const T* in, T* x, T* tmp, hipStream_t st) const {
     int rc = vcycle<DIM, NB, T>(cy, 0, in, tmp, st);
@@ -1758,6 +2159,18 @@ struct BlockPrec {
   T* vu;  // n_rows * (NB-1)
   T* zu;  // n_rows * (NB-1)
   long n;
+  void key(std::vector<char>& k) const {
+    put_key(k, 3);
+    mg::cycle_key(cy, k);
+    put_key(k, n_rows);
+    put_key(k, rp);
+    put_key(k, ci);
+    put_key(k, vals);
+    put_key(k, sinv);
+    put_key(k, vu);
+    put_key(k, zu);
+    put_key(k, n);
+  }
   int apply(const T* in, T* out, hipStream_t st) const {
     int rb = gm_blocks(n_rows, 256);
     k_split<NB, T><<<rb, 256, 0, st>>>(n_rows, in, vu);

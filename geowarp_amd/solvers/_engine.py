@@ -242,7 +242,8 @@ class Engine:
     def solve_mass(self, mrhs, tol=1e-13, maxiter=20000):
         """Consistent-mass projection M p = rhs (the reference solves it with
         pyamg SA-BiCGStab (qc2d:288-293) or Warp Jacobi-BiCGStab (qc3d:410-412)
-        at 1e-8; here to `tol`)."""
+        at 1e-8; here to `tol`): Jacobi-preconditioned CG, the matrix being
+        symmetric positive definite, with Jacobi-BiCGStab as the fallback."""
         lib = L.lib()
         L.check(lib.gm_fix_empty_diag(self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx), L.ptr(self.mvals),
                                       L.ptr(mrhs), _stream()), 'gm_fix_empty_diag')
@@ -250,9 +251,15 @@ class Engine:
         ws = self.workspace(1, False)
         it = L.c_int(0)
         rr = L.c_double(0.0)
-        L.check(lib.gm_bicgstab(1, 0, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx), L.ptr(self.mvals),
-                                L.ptr(mrhs), L.ptr(xp), L.ctypes.byref(ws), tol, maxiter, 4, L.ctypes.byref(it),
-                                L.ctypes.byref(rr), _stream()), 'gm_bicgstab(mass)')
+        args = (1, 0, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx), L.ptr(self.mvals), L.ptr(mrhs),
+                L.ptr(xp), L.ctypes.byref(ws), tol, maxiter, 4, L.ctypes.byref(it), L.ctypes.byref(rr), _stream())
+        L.check(lib.gm_pcg(*args), 'gm_pcg(mass)')
+        self.mass_solver = 'pcg'
+        if not (rr.value <= tol):
+            its = it.value
+            L.check(lib.gm_bicgstab(*args), 'gm_bicgstab(mass)')
+            it.value += its
+            self.mass_solver = 'bicgstab'
         self.mass_iters = it.value
         self.mass_relres = rr.value
         return xp
@@ -316,6 +323,32 @@ class Engine:
         self.stats['spmv_tune'] = dict(threads=best.value, ms={t: round(v, 5) for t, v in
                                                                zip((64, 128, 256, 512), list(ms))}, nnzb=self.nnzb)
         return best.value
+
+    def autotune_jacobian(self, reps=3):
+        """Pick the per-bin and gather kernel variants of the binned 3D u-p Jacobian
+        on the current tangent (once per engine and precision; every variant writes
+        the same bits); the timings are kept in stats['jacobian_tune']."""
+        if not (self.dim == 3 and self.coupled and self.jacobian_kernel == 'bins'):
+            self.stats['jacobian_tune'] = False  # nothing to choose for this form
+            return None
+        fp32 = self.precision == 'mixed'
+        t32 = fp32 and self._tangent_fp32()
+        vals, v32 = (self.vals64, False) if (fp32 and not t32) else (self.vals, fp32)
+        g, b, pat = L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ctypes.byref(self.pattern)
+        need = int(L.lib().gm_node_jacobian_work_bytes(g, int(t32)))
+        w = self._bins_workspace(need)
+        bb, bg = L.c_int(0), L.c_int(0)
+        ms = (L.ctypes.c_float * 7)()
+        L.check(L.lib().gm_jacobian_autotune(g, b, pat, L.ptr(self.D), int(t32), L.ptr(self.freem), L.ptr(vals),
+                                             int(v32), L.ptr(w), need, reps, L.ctypes.byref(bb),
+                                             L.ctypes.byref(bg), ms, _stream()), 'gm_jacobian_autotune')
+        if fp32 and not t32:
+            L.check(L.lib().gm_d2f(self.nnzb * self.nb * self.nb, L.ptr(self.vals64), L.ptr(self.vals), _stream()),
+                    'gm_d2f')
+        self.stats['jacobian_tune'] = dict(bins_variant=bb.value, gather_variant=bg.value,
+                                           bins_ms=[round(v, 4) for v in list(ms)[:4]],
+                                           gather_ms=[round(v, 4) for v in list(ms)[4:]])
+        return bb.value, bg.value
 
     def jacobian(self, x, lp):
         fp32 = self.precision == 'mixed'

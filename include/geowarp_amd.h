@@ -111,6 +111,29 @@ int gm_jacobian_bins(const gm_grid* g, const gm_bins* bins, const void* D, int t
                      size_t work_bytes, hipStream_t stream);
 int gm_jacobian_gather(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, int tangent_fp32,
                        const void* work, const uint8_t* freem, void* vals, int vals_fp32, hipStream_t stream);
+/* The same phases with an explicit kernel variant (the two above use the one the
+   tuner below chose for the tangent's precision, else variant 0).  Per-bin variants
+   0-3 (particles per group, workgroups per CU, grid): FP32 (8, 2, one per bin),
+   (8, 2, persistent), (4, 4, persistent), (4, 3, persistent); FP64 (4, 2, one per
+   bin), (4, 2, persistent), (8, 1, one per bin), (8, 1, persistent).  Gather
+   variants 0 (per-lane reads of the workspace), 1 and 2 (bin chunks staged in LDS,
+   27 or 9 at a time in FP32, 9 or 3 in FP64).
+   Every variant writes the same bits; -1 for a variant out of range. */
+int gm_jacobian_bins_variant(const gm_grid* g, const gm_bins* bins, const void* D, int tangent_fp32, int variant,
+                             void* work, size_t work_bytes, hipStream_t stream);
+int gm_jacobian_gather_variant(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, int tangent_fp32,
+                               int variant, const void* work, const uint8_t* freem, void* vals, int vals_fp32,
+                               hipStream_t stream);
+/* Tile and launch-shape tuner of the binned assembly: times every per-bin and
+   gather variant on this tangent (`reps` launches each after a warm-up, HIP events
+   on the stream; ms[0..3] per-bin, ms[4..6] gather, milliseconds per launch), keeps
+   the fastest of each for later assemblies in this precision in this process and
+   returns them.  vals and work end holding the assembly.  No reference counterpart:
+   the reference's Jacobian (qc3d:285-321) has no launch parameters to choose. */
+int gm_jacobian_autotune(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, const void* D,
+                         int tangent_fp32, const uint8_t* freem, void* vals, int vals_fp32, void* work,
+                         size_t work_bytes, int reps, int* best_bins, int* best_gather, float* ms,
+                         hipStream_t stream);
 
 /* Load-step sensitivity of the 3D u-p Neo-Hookean form to Young's modulus: per
    particle (bin-sorted row kq, particle sorted[kq]) the derivative of its stress
@@ -182,6 +205,16 @@ int gm_update_lp(int n, int dim, const double* F_new, double* lp, double lp0, co
 int gm_bicgstab(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
                 const void* b, void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int* iters,
                 double* relres, hipStream_t stream);
+/* Block-Jacobi preconditioned conjugate gradients for symmetric positive definite
+   node block-CSR systems (the consistent-mass projection), same arguments, workspace
+   and stopping rule as gm_bicgstab: one SpMV and two reductions per iteration, every
+   scalar on the device.  Stops early, with relres above tol, when p.Ap is not
+   positive (the matrix is not positive definite).  Replaces the reference's
+   Jacobi-BiCGStab of the mass projection (qc3d:410-412, qc2d:288-293). */
+int gm_pcg(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals, const void* b,
+           void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int* iters, double* relres,
+           hipStream_t stream);
+
 
 /* Restarted right-preconditioned GMRES(m) with block-Jacobi and classical Gram-Schmidt
    Arnoldi (reorthogonalised when a pass removes more than half of the vector's norm) on
@@ -322,6 +355,15 @@ int gm_gmres_mg(int dim, int nb, int fp32, int n_levels, const gm_mg_level* leve
                 int nu_pre, int nu_post, int coarse_sweeps, double omega, const void* b, void* x, void* V, void* Z,
                 void* z, void* w, double* part, double* dh, int m, double tol, int maxiter, int* iters,
                 double* relres, hipStream_t stream);
+/* Arnoldi step graphs of the GMRES solvers (gm_gmres, gm_gmres_mg, gm_gmres_up): each
+   step's launches (preconditioner, operator, Gram-Schmidt passes, normalisation, copy of
+   the Hessenberg column to pinned host memory) are captured once per step index as a HIP
+   graph and replayed, keyed on every pointer and size they use.  on = 1 / 0 turns them on
+   or off for later solves (default on; GM_KRYLOV_GRAPHS=0 in the environment starts off),
+   -1 only reads the setting; returns the previous setting.  Both give the same bits.  No
+   reference counterpart (the reference's solve is Warp's bicgstab, qc3d:335-338). */
+int gm_krylov_graphs(int on);
+
 
 /* u-p systems (nb = dim + 1): block lower-triangular preconditioner
    z_u = MG(A_uu)^-1 v_u, z_p = S^-1 (v_p - K_pu z_u), S the diagonal Schur

@@ -88,3 +88,52 @@ def test_binned_residual_equals_node_row_residual():
     rel = float((a - b).norm() / a.norm())
     print('residual bins vs rows', rel, 'norm', float(a.norm()))
     assert float(a.norm()) > 0 and rel < 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_binned_jacobian_variants_write_the_same_bits(precision):
+    """Every per-bin variant of the binned assembly (particles per group, workgroups
+    per CU, one workgroup per bin or a persistent sweep with prefetch) and both
+    gather variants (per-lane reads, chunks staged in LDS in two round sizes) give the same matrix bit
+    for bit (groups of 4 particles split every bin in two or more); the workspace and the matrix are filled with NaN before each pair, so
+    an entry a variant fails to write shows.  The tuner returns variants in range
+    and leaves the same matrix."""
+    import torch
+    from geowarp_amd import _lib as L
+    from geowarp_amd.solvers._engine import _stream
+    s = _deformed(precision)
+    eng = s.engine
+    lib = L.lib()
+    t32 = precision == 'mixed' and eng._tangent_fp32()
+    v32 = precision == 'mixed'
+    g, b, pat = L.ctypes.byref(eng.grid), L.ctypes.byref(eng.bins), L.ctypes.byref(eng.pattern)
+    need = int(lib.gm_node_jacobian_work_bytes(g, int(t32)))
+    w = eng._bins_workspace(need)
+    n = eng.nnzb * eng.nb * eng.nb
+    vals = torch.empty(n, dtype=torch.float32 if v32 else torch.float64, device=eng.vals.device)
+    counts = eng.cell_start[1:] - eng.cell_start[:-1]
+    print(precision, 'bins', int((counts > 0).sum()), 'max particles per bin', int(counts.max()))
+    ref = None
+    for vb in range(4):
+        for vg in range(3):
+            w.fill_(255)
+            vals.fill_(float('nan'))
+            L.check(lib.gm_jacobian_bins_variant(g, b, L.ptr(eng.D), int(t32), vb, L.ptr(w), need, _stream()),
+                    'gm_jacobian_bins_variant')
+            L.check(lib.gm_jacobian_gather_variant(g, b, pat, int(t32), vg, L.ptr(w), L.ptr(eng.freem),
+                                                   L.ptr(vals), int(v32), _stream()), 'gm_jacobian_gather_variant')
+            out = vals.clone()
+            assert not bool(torch.isnan(out).any()), (vb, vg)
+            if ref is None:
+                ref = out
+            else:
+                assert torch.equal(out, ref), (vb, vg)
+    with pytest.raises(L.GMError):
+        L.check(lib.gm_jacobian_bins_variant(g, b, L.ptr(eng.D), int(t32), 4, L.ptr(w), need, _stream()), 'variant')
+    bb, bg = eng.autotune_jacobian()
+    torch.cuda.synchronize()
+    print(precision, eng.stats['jacobian_tune'])
+    assert 0 <= bb < 4 and 0 <= bg < 3
+    got = eng.vals64 if (v32 and not t32) else eng.vals
+    assert torch.equal(got[:n], ref.to(got.dtype))
