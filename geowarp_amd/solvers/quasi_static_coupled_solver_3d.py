@@ -178,12 +178,7 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
         self._newton(current_step, total_steps, self._new, self._old, args, after_residual=after_residual)
         if self.step_sensitivity:
             self._step_E_sensitivity(total_steps, args)
-        eng.g2p(self.x_particles, self.GIMP_lp, self._new, self._old, self.deformation_gradient_total_old,
-                self.deformation_gradient_total_new, self.left_Cauchy_Green_old, self.left_Cauchy_Green_new,
-                self.plastic_strain_old, self.plastic_strain_new, self.particle_pressure_array)
-        eng.update_lp(self.deformation_gradient_total_new, self.GIMP_lp, self.GIMP_lp_initial, self.x_particles,
-                      (self.domain_min_x, self.domain_min_y, self.domain_min_z),
-                      (self.domain_max_x, self.domain_max_y, self.domain_max_z))
+        self._update_particles()
         self.indentation_force_list = np.append(self.indentation_force_list, float(self.indentation_force.item()))
 
     def _step_E_sensitivity(self, total_steps, args, tol=1e-12):
@@ -249,6 +244,113 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
         self.E_grad_step_explicit = dLdF * F / E
         self.E_grad_step = self.E_grad_step_explicit + float(torch.dot(lam, rE).item())
         self.sensitivity_stats = dict(krylov_iters=its, relres=rel, force=F)
+
+    def _set_E(self, E):
+        self.youngs_modulus = float(E)
+        self._set_lame()
+
+    def _update_particles(self):
+        """G2P and the particle-domain update at the current nodal solution (the end of
+        newton_iter, qc3d:378-392)."""
+        eng = self.engine
+        eng.g2p(self.x_particles, self.GIMP_lp, self._new, self._old, self.deformation_gradient_total_old,
+                self.deformation_gradient_total_new, self.left_Cauchy_Green_old, self.left_Cauchy_Green_new,
+                self.plastic_strain_old, self.plastic_strain_new, self.particle_pressure_array)
+        eng.update_lp(self.deformation_gradient_total_new, self.GIMP_lp, self.GIMP_lp_initial, self.x_particles,
+                      (self.domain_min_x, self.domain_min_y, self.domain_min_z),
+                      (self.domain_max_x, self.domain_max_y, self.domain_max_z))
+
+    def path_E_gradient(self, n_steps, total_steps, h_rel=1e-6, tol=1e-12):
+        """Total derivative of the indentation loss after load steps 0 .. n_steps-1 with
+        respect to Young's modulus through the whole load path, i.e. including the
+        particle state (positions, domains, deformation gradients, stresses, pore
+        pressures) every load step hands to the next; the reference's tape covers only
+        the last step's residual (qc3d:322-325, 3d_indentation_coupled_cpGIMP.py:195-232).
+
+        Forward (tangent-linear) sensitivity along the forward run, from the current
+        state: for every load step k with initial state s and converged nodal solution u,
+            K du/dE = dR/ds . ds/dE + dR/dE          (K = -dR/du, converged FP64 Jacobian)
+            ds'/dE = dG/du . du/dE + dG/ds . ds/dE + dG/dE
+        with G the load step's particle update (P2G projection, stage, G2P, domain
+        update).  The products with dR/ds, dR/dE and with G's derivatives are central
+        differences of the device residual and update in the tangent's own direction
+        (E + t, s + t ds/dE, u + t du/dE, t = h_rel E): one pair of evaluations each, at
+        the cost of O(t^2) truncation.  At the last step the indentation force is
+        differentiated the same way.  FP64 form; each step must converge in one part.
+        Returns dL/dE (also in self.E_grad_path, the force derivative in
+        self.path_force_derivative) and leaves the solver at the end of the path."""
+        eng = self.engine
+        if eng.precision != 'fp64':
+            raise L.GMError('the load-path sensitivity needs the FP64 form')
+        E = float(self.youngs_modulus)
+        t = h_rel * E
+        keys = [k for k in self._STATE + self._STATE_EXTRA if getattr(self, k).dtype == torch.float64]
+        ds = {k: torch.zeros_like(getattr(self, k)) for k in keys}
+        x, lp = self.x_particles, self.GIMP_lp
+
+        def shifted(state, sg):
+            return {k: (v + (sg * t) * ds[k]) if k in ds else v for k, v in state.items()}
+
+        def at(k, state, sg, u):
+            # perturbed initial state and modulus, the step's P2G projection, nodal solution u
+            self._set_E(E + sg * t)
+            self.load_state_dict(shifted(state, sg))
+            self.prepare_step(k, total_steps)
+            self._new.copy_(u)
+            return self._args(k, total_steps)
+
+        dF = 0.0
+        its_total = 0
+        for k in range(n_steps):
+            s0 = self.state_dict()
+            self.advance_one_step(k, total_steps)
+            if not self.newton_converged or self.substep_count != 1:
+                raise L.GMError('load-path sensitivity: load step %d did not converge in one part' % k)
+            u = self._new.clone()
+            s1 = self.state_dict()
+            force = float(self.indentation_force.item())
+            try:
+                # dR/ds . ds/dE + dR/dE at u
+                r = []
+                for sg in (1.0, -1.0):
+                    args = at(k, s0, sg, u)
+                    eng.stage(args, want_tangent=False)
+                    eng.residual(x, lp)
+                    r.append(eng.rhs.clone())
+                rhs = (r[0] - r[1]) / (2.0 * t)
+                # the converged Jacobian at the unperturbed state
+                args = at(k, s0, 0.0, u)
+                eng.stage(args, want_tangent=True)
+                eng.residual(x, lp)
+                eng.jacobian(x, lp)
+                du = torch.zeros_like(rhs)
+                its, rel = eng.krylov(rhs, du, False, tol, self.linear_maxiter,
+                                      self.krylov_method if self.krylov_method in ('mg', 'gmres') else 'gmres',
+                                      self.krylov_restart)
+                its_total += its
+                # the state the step hands on, differentiated along (E, s, u)
+                g = []
+                for sg in (1.0, -1.0):
+                    args = at(k, s0, sg, u + (sg * t) * du)
+                    eng.stage(args, want_tangent=False)
+                    if k == n_steps - 1:
+                        eng.residual(x, lp)
+                        eng.sum_into(eng.pen_force, 1)
+                        g.append(float(eng.scal[1].item()))
+                    self._update_particles()
+                    g.append({kk: getattr(self, kk).clone() for kk in keys})
+                if k == n_steps - 1:
+                    dF = (g[0] - g[2]) / (2.0 * t)
+                    g = [g[1], g[3]]
+                ds = {kk: (g[0][kk] - g[1][kk]) / (2.0 * t) for kk in keys}
+            finally:
+                self._set_E(E)
+                self.load_state_dict(s1)
+        S = abs(self.footing_incr_disp) * float(total_steps)
+        self.path_force_derivative = dF
+        self.E_grad_path = 2.0 * (force / S - TARGET_FORCE) / S * dF
+        self.path_stats = dict(krylov_iters=its_total, force=force)
+        return self.E_grad_path
 
     def prepare_step(self, current_step, total_steps):
         """reset, boundary callback, P2G and consistent-mass projection (qc3d:397-417)."""

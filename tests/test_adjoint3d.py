@@ -79,3 +79,39 @@ def test_step_sensitivity_needs_fp64_and_neo_hookean():
     s = H.make(gs, cfg)
     with pytest.raises(GMError):
         s.advance_one_step(0, cfg['total_steps'])
+
+
+@pytest.mark.gpu
+def test_load_path_E_gradient_matches_central_fd():
+    """The total dL/dE through three load steps (path_E_gradient: the state each step
+    hands to the next is differentiated too) against central differences of the whole
+    forward run in E; the single-step derivative of the last step, which holds that
+    step's initial state fixed, differs from it."""
+    import geowarp_amd.solvers as gs
+    n = 3
+    cfg = _cfg()
+    total = cfg['total_steps']
+    E = cfg['material_dict']['youngs_modulus']
+    s = H.make(gs, cfg)
+    grad = s.path_E_gradient(n, total)
+    dF = s.path_force_derivative
+    h = 1e-5 * E
+    out = []
+    for Ep in (E + h, E - h):
+        t = H.make(gs, _cfg(E=Ep))
+        for k in range(n):
+            t.advance_one_step(k, total)
+            assert t.newton_converged
+        out.append(_force_and_loss(t))
+    dF_fd = (out[0][0] - out[1][0]) / (2 * h)
+    dL_fd = (out[0][1] - out[1][1]) / (2 * h)
+    # the last step alone, from the same state at its start
+    s2 = H.make(gs, _cfg(sens=True))
+    for k in range(n):
+        s2.advance_one_step(k, total)
+    print('path dF/dE %.9e fd %.9e | dL/dE path %.9e fd %.9e | last step alone %.9e' %
+          (dF, dF_fd, grad, dL_fd, s2.E_grad_step), s.path_stats)
+    assert abs(dF_fd) > 0
+    assert abs(dF - dF_fd) <= REL * abs(dF_fd)
+    assert abs(grad - dL_fd) <= REL * abs(dL_fd)
+    assert abs(s2.E_grad_step - dL_fd) > 10 * REL * abs(dL_fd)
