@@ -119,7 +119,7 @@ _SIGS = {
     'gm_bicgstab': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, ctypes.POINTER(gm_krylov_ws), c_double, c_int,
                             c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_double), _P]),
     'gm_pcg': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, ctypes.POINTER(gm_krylov_ws), c_double, c_int,
-                            c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_double), _P]),
+                       c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_double), _P]),
     'gm_gmres': (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, c_int, c_double, c_int,
                          ctypes.POINTER(c_int), ctypes.POINTER(c_double), _P]),
     'gm_bsr_spmv': (c_int, [c_int, c_int, c_int, c_int, _P, _P, _P, _P, _P, _P]),

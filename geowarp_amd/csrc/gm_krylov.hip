@@ -757,9 +757,22 @@ __global__ void k_pcg_p(long n, const double* sc, const TV* z, TV* p) {
     p[i] = (TV)((double)z[i] + beta * (double)p[i]);
 }
 
+// lumped (row-sum) diagonal preconditioner of a scalar matrix: dinv = 1 / sum_k A_ik
+// (a consistent-mass matrix has non-negative entries and row sums equal to the lumped
+// nodal masses)
+template <typename TM>
+__global__ void k_rowsum_inv(int n_rows, const int* rp, const TM* vals, TM* dinv) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n_rows) return;
+  double s = 0.0;
+  for (int k = rp[row]; k < rp[row + 1]; ++k) s += (double)vals[k];
+  dinv[row] = (TM)((s != 0.0) ? 1.0 / s : 1.0);
+}
+
 template <int NB, typename TM, typename TV>
 static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, const TV* b, TV* x, gm_krylov_ws* ws,
-                    double tol, int maxiter, int check_every, int* iters_out, double* relres_out, hipStream_t st) {
+                    double tol, int maxiter, int check_every, int lumped, int* iters_out, double* relres_out,
+                    hipStream_t st) {
   long n = (long)n_rows * NB;
   TV *r = (TV*)ws->r, *p = (TV*)ws->p, *q = (TV*)ws->v, *z = (TV*)ws->z;
   TM* dinv = (TM*)ws->dinv;
@@ -768,7 +781,10 @@ static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, co
   const int rb = gm_blocks(n_rows, 256);
   const int db = std::min(1024, gm_blocks(n_rows, RED_T));
   const int sb = spmv_blocks(n_rows, NB);
-  k_diag_inv<NB, TM><<<rb, 256, 0, st>>>(n_rows, rp, ci, vals, dinv);
+  if (NB == 1 && lumped)
+    k_rowsum_inv<TM><<<rb, 256, 0, st>>>(n_rows, rp, vals, dinv);
+  else
+    k_diag_inv<NB, TM><<<rb, 256, 0, st>>>(n_rows, rp, ci, vals, dinv);
   k_pcg_start<NB, TM, TV><<<db, RED_T, 0, st>>>(n_rows, b, dinv, x, r, z, p, part);
   k_sum_parts<<<1, RED_T, 0, st>>>(part, db, sc + C_RZ, sc + C_RR);
   k_pcg_init_sc<<<1, 1, 0, st>>>(sc);
@@ -816,16 +832,17 @@ static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, co
 }
 
 extern "C" int gm_pcg(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
-                      const void* b, void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int* iters,
-                      double* relres, hipStream_t s) {
+                      const void* b, void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int lumped,
+                      int* iters, double* relres, hipStream_t s) {
   GM_REQUIRE(ws && iters && relres, "gm_pcg: null argument");
+  GM_REQUIRE(!lumped || nb == 1, "gm_pcg: the lumped preconditioner needs a scalar matrix (nb = 1)");
   if (check_every < 1) check_every = 1;
 #define GM_PCG(NB)                                                                                              \
   if (nb == NB)                                                                                                 \
     return fp32 ? pcg_impl<NB, float, float>(n_rows, row_ptr, col_idx, (const float*)vals, (const float*)b,     \
-                                             (float*)x, ws, tol, maxiter, check_every, iters, relres, s)        \
+                                             (float*)x, ws, tol, maxiter, check_every, lumped, iters, relres, s) \
                 : pcg_impl<NB, double, double>(n_rows, row_ptr, col_idx, (const double*)vals, (const double*)b, \
-                                               (double*)x, ws, tol, maxiter, check_every, iters, relres, s);
+                                               (double*)x, ws, tol, maxiter, check_every, lumped, iters, relres, s);
   GM_PCG(1)
   GM_PCG(2)
   GM_PCG(3)
