@@ -650,22 +650,31 @@ extern "C" int gm_bicgstab(int nb, int fp32, int n_rows, const int* row_ptr, con
 }
 
 // ------------------------------------------------------------------ PCG
-// Conjugate gradients (Hestenes & Stiefel 1952) with the block-Jacobi preconditioner
-// for symmetric positive definite systems (the consistent-mass projection): one SpMV
-// with the fused p.Ap partials and two reductions per iteration, every scalar on the
-// device; the host reads ||r|| every `check_every` iterations.
+// Conjugate gradients (Hestenes & Stiefel 1952) with a diagonal-block preconditioner for
+// symmetric positive definite systems (the consistent-mass projection): per iteration
+// the SpMV q = A p with fused p.q partials, one workgroup summing them into alpha, the
+// update x += alpha p, r -= alpha q, z = M^-1 r with fused r.z and r.r partials, one
+// workgroup summing them into beta, and p = z + beta p; every scalar stays on the
+// device and the host reads ||r|| every `check_every` iterations.
 enum { C_RZ = 0, C_RZN, C_PQ, C_RR, C_BB, C_ALPHA, C_BETA, C_FLAG, C_NSC };
 
-// x = 0, r = b, z = Dinv b, p = z; per-workgroup partials of r.z and r.r
+// x = 0 (cold start) or kept (warm start, ax = A x), r = b - ax, z = Dinv r, p = z;
+// per-workgroup partials of r.z and r.r
 template <int NB, typename TM, typename TV>
-__global__ void k_pcg_start(int n_rows, const TV* b, const TM* dinv, TV* x, TV* r, TV* z, TV* p, double* part) {
+__global__ void k_pcg_start(int n_rows, const TV* b, const TV* ax, const TM* dinv, TV* x, TV* r, TV* z, TV* p,
+                            double* part) {
   __shared__ double a[RED_T], c[RED_T];
   double s0 = 0.0, s1 = 0.0;
   for (int row = blockIdx.x * RED_T + threadIdx.x; row < n_rows; row += gridDim.x * RED_T) {
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-      x[row * NB + j] = (TV)0;
-      r[row * NB + j] = b[row * NB + j];
+      const int i = row * NB + j;
+      if (ax) {
+        r[i] = (TV)((double)b[i] - (double)ax[i]);
+      } else {
+        x[i] = (TV)0;
+        r[i] = b[i];
+      }
     }
     apply_dinv<NB, TM, TV>(dinv, row, r, z);
 #pragma unroll
@@ -692,23 +701,60 @@ __global__ void k_pcg_start(int n_rows, const TV* b, const TM* dinv, TV* x, TV* 
   }
 }
 
-__global__ void k_pcg_init_sc(double* sc) {
-  sc[C_BB] = sc[C_RR];
-  sc[C_FLAG] = 0.0;
+__global__ void k_pcg_init_sc(double* sc) { sc[C_FLAG] = 0.0; }
+
+// one workgroup: the sums of n partial pairs in the order of k_sum_parts
+__device__ void sum_pairs(const double* part, int n, double* s0, double* s1) {
+  __shared__ double a[RED_T], b[RED_T];
+  double u = 0.0, v = 0.0;
+  for (int i = threadIdx.x; i < n; i += RED_T) {
+    u += part[2 * i];
+    v += part[2 * i + 1];
+  }
+  a[threadIdx.x] = u;
+  b[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = RED_T / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      a[threadIdx.x] += a[threadIdx.x + o];
+      b[threadIdx.x] += b[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  *s0 = a[0];
+  *s1 = b[0];
 }
 
-// alpha = (r.z) / (p.Ap); a non-positive or non-finite p.Ap stops the iteration (flag)
-__global__ void k_pcg_alpha(double* sc) {
-  const double pq = sc[C_PQ];
-  const bool ok = pq > 0.0 && pq < 1e308 && sc[C_FLAG] == 0.0;
-  sc[C_ALPHA] = ok ? sc[C_RZ] / pq : 0.0;
-  if (!ok) sc[C_FLAG] = 1.0;
+// p.q from the SpMV's partials; alpha = (r.z) / (p.q); a non-positive or non-finite
+// p.q stops the iteration (flag, alpha = 0)
+__global__ void __launch_bounds__(RED_T) k_pcg_alpha(const double* part, int n, double* sc) {
+  double pq, unused;
+  sum_pairs(part, n, &pq, &unused);
+  if (threadIdx.x == 0) {
+    sc[C_PQ] = pq;
+    const bool ok = pq > 0.0 && pq < 1e308 && sc[C_FLAG] == 0.0;
+    sc[C_ALPHA] = ok ? sc[C_RZ] / pq : 0.0;
+    if (!ok) sc[C_FLAG] = 1.0;
+  }
+}
+
+// r.z and r.r from the update's partials; beta = (r.z)_new / (r.z)
+__global__ void __launch_bounds__(RED_T) k_pcg_beta(const double* part, int n, double* sc) {
+  double rzn, rr;
+  sum_pairs(part, n, &rzn, &rr);
+  if (threadIdx.x == 0) {
+    const double rz = sc[C_RZ];
+    sc[C_RZN] = rzn;
+    sc[C_RR] = rr;
+    sc[C_BETA] = (rz != 0.0) ? rzn / rz : 0.0;
+    sc[C_RZ] = rzn;
+  }
 }
 
 // x += alpha p, r -= alpha q, z = Dinv r; per-workgroup partials of r.z and r.r
 template <int NB, typename TM, typename TV>
-__global__ void k_pcg_xr(int n_rows, const double* sc, TV* x, TV* r, const TV* p, const TV* q, const TM* dinv, TV* z,
-                         double* part) {
+__global__ void k_pcg_xr(int n_rows, const double* sc, TV* x, TV* r, const TV* p, const TV* q, const TM* dinv,
+                         TV* z, double* part) {
   __shared__ double a[RED_T], c[RED_T];
   const double alpha = sc[C_ALPHA];
   double s0 = 0.0, s1 = 0.0;
@@ -743,13 +789,6 @@ __global__ void k_pcg_xr(int n_rows, const double* sc, TV* x, TV* r, const TV* p
   }
 }
 
-// beta = (r.z)_new / (r.z); keeps (r.z)_new for the next step
-__global__ void k_pcg_beta(double* sc) {
-  const double rz = sc[C_RZ];
-  sc[C_BETA] = (rz != 0.0) ? sc[C_RZN] / rz : 0.0;
-  sc[C_RZ] = sc[C_RZN];
-}
-
 template <typename TV>
 __global__ void k_pcg_p(long n, const double* sc, const TV* z, TV* p) {
   const double beta = sc[C_BETA];
@@ -771,8 +810,9 @@ __global__ void k_rowsum_inv(int n_rows, const int* rp, const TM* vals, TM* dinv
 
 template <int NB, typename TM, typename TV>
 static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, const TV* b, TV* x, gm_krylov_ws* ws,
-                    double tol, int maxiter, int check_every, int lumped, int* iters_out, double* relres_out,
+                    double tol, int maxiter, int check_every, int options, int* iters_out, double* relres_out,
                     hipStream_t st) {
+  const bool lumped = options & 1, warm = options & 2;
   long n = (long)n_rows * NB;
   TV *r = (TV*)ws->r, *p = (TV*)ws->p, *q = (TV*)ws->v, *z = (TV*)ws->z;
   TM* dinv = (TM*)ws->dinv;
@@ -785,9 +825,12 @@ static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, co
     k_rowsum_inv<TM><<<rb, 256, 0, st>>>(n_rows, rp, vals, dinv);
   else
     k_diag_inv<NB, TM><<<rb, 256, 0, st>>>(n_rows, rp, ci, vals, dinv);
-  k_pcg_start<NB, TM, TV><<<db, RED_T, 0, st>>>(n_rows, b, dinv, x, r, z, p, part);
-  k_sum_parts<<<1, RED_T, 0, st>>>(part, db, sc + C_RZ, sc + C_RR);
   k_pcg_init_sc<<<1, 1, 0, st>>>(sc);
+  k_dot2<TV><<<dot_blocks(n), RED_T, 0, st>>>(n, b, b, nullptr, nullptr, part);
+  k_sum_parts<<<1, RED_T, 0, st>>>(part, dot_blocks(n), sc + C_BB, nullptr);
+  if (warm) launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, x, q, nullptr, nullptr, nullptr, st);
+  k_pcg_start<NB, TM, TV><<<db, RED_T, 0, st>>>(n_rows, b, warm ? q : nullptr, dinv, x, r, z, p, part);
+  k_sum_parts<<<1, RED_T, 0, st>>>(part, db, sc + C_RZ, sc + C_RR);
   GM_LAUNCH_CHECK();
   double hs[C_NSC];
   GM_HIP(hipMemcpyAsync(hs, sc, sizeof(double) * C_NSC, hipMemcpyDeviceToHost, st));
@@ -799,18 +842,20 @@ static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, co
     gm::set_error("pcg: right-hand side is not finite");
     return -5;
   }
-  if (bnorm == 0.0) return 0;
+  if (bnorm == 0.0) {
+    if (warm) k_copy_zero<TV><<<dot_blocks(n), RED_T, 0, st>>>(n, b, r, r, r, r, x);  // x = 0
+    GM_LAUNCH_CHECK();
+    return 0;
+  }
+  double rel = sqrt(fmax(hs[C_RR], 0.0)) / bnorm;
   int it = 0;
-  double rel = 1.0;
-  while (it < maxiter) {
+  while (it < maxiter && rel > tol) {
     const int nrun = std::min(check_every, maxiter - it);
     for (int k = 0; k < nrun; ++k) {
       launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, p, q, p, nullptr, part, st);
-      k_sum_parts<<<1, RED_T, 0, st>>>(part, sb, sc + C_PQ, nullptr);
-      k_pcg_alpha<<<1, 1, 0, st>>>(sc);
+      k_pcg_alpha<<<1, RED_T, 0, st>>>(part, sb, sc);
       k_pcg_xr<NB, TM, TV><<<db, RED_T, 0, st>>>(n_rows, sc, x, r, p, q, dinv, z, part);
-      k_sum_parts<<<1, RED_T, 0, st>>>(part, db, sc + C_RZN, sc + C_RR);
-      k_pcg_beta<<<1, 1, 0, st>>>(sc);
+      k_pcg_beta<<<1, RED_T, 0, st>>>(part, db, sc);
       k_pcg_p<TV><<<gm_blocks(n, 256), 256, 0, st>>>(n, sc, z, p);
     }
     GM_LAUNCH_CHECK();
@@ -824,7 +869,7 @@ static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, co
       *relres_out = hs[C_RR];
       return -5;
     }
-    if (rel <= tol || hs[C_FLAG] != 0.0) break;  // converged, or p.Ap <= 0: the caller sees rel
+    if (hs[C_FLAG] != 0.0) break;  // p.Ap <= 0: the caller sees rel
   }
   *iters_out = it;
   *relres_out = rel;
@@ -832,10 +877,11 @@ static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, co
 }
 
 extern "C" int gm_pcg(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals,
-                      const void* b, void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int lumped,
+                      const void* b, void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int options,
                       int* iters, double* relres, hipStream_t s) {
   GM_REQUIRE(ws && iters && relres, "gm_pcg: null argument");
-  GM_REQUIRE(!lumped || nb == 1, "gm_pcg: the lumped preconditioner needs a scalar matrix (nb = 1)");
+  GM_REQUIRE(!(options & 1) || nb == 1, "gm_pcg: the lumped preconditioner needs a scalar matrix (nb = 1)");
+  const int lumped = options;
   if (check_every < 1) check_every = 1;
 #define GM_PCG(NB)                                                                                              \
   if (nb == NB)                                                                                                 \

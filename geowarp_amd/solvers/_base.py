@@ -185,6 +185,8 @@ class _QuasiStaticBase:
         self.engine.mixed_assembly = it.get('mixed_assembly', 'fp32')
         self.engine.mg_lag = bool(it.get('mg_lag', True))
         self.engine.mg_options = mg_opts
+        # coupled forms: start each consistent-mass projection from the last nodal pressures
+        self.warm_mass = bool(it.get('warm_mass', True))
         dev = self.device
         self.dofStruct = DofStruct(self.n_matrix_size, dev)
         if x_particles is None:

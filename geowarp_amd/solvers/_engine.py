@@ -242,27 +242,31 @@ class Engine:
                                  _stream()), 'gm_free_mask')
         return mrhs
 
-    def solve_mass(self, mrhs, tol=1e-13, maxiter=20000):
+    def solve_mass(self, mrhs, tol=1e-13, maxiter=20000, x0=None):
         """Consistent-mass projection M p = rhs (the reference solves it with
         pyamg SA-BiCGStab (qc2d:288-293) or Warp Jacobi-BiCGStab (qc3d:410-412)
         at 1e-8; here to `tol`): conjugate gradients, the matrix being symmetric
         positive definite, preconditioned by its diagonal (mass_lumped_pc: by the
-        lumped masses, its row sums), with Jacobi-BiCGStab as the fallback."""
+        lumped masses, its row sums), from x0 when given (e.g. the previous load
+        step's nodal pressures), with Jacobi-BiCGStab as the fallback."""
         lib = L.lib()
         L.check(lib.gm_fix_empty_diag(self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx), L.ptr(self.mvals),
                                       L.ptr(mrhs), _stream()), 'gm_fix_empty_diag')
-        xp = torch.zeros(self.n_nodes, dtype=torch.float64, device=self.dev)
+        if x0 is not None:
+            xp = x0.to(torch.float64).clone()
+        else:
+            xp = torch.zeros(self.n_nodes, dtype=torch.float64, device=self.dev)
         ws = self.workspace(1, False)
         it = L.c_int(0)
         rr = L.c_double(0.0)
         head = (1, 0, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx), L.ptr(self.mvals), L.ptr(mrhs),
                 L.ptr(xp), L.ctypes.byref(ws), tol, maxiter, 4)
         tail = (L.ctypes.byref(it), L.ctypes.byref(rr), _stream())
-        L.check(lib.gm_pcg(*head, int(self.mass_lumped_pc), *tail), 'gm_pcg(mass)')
+        L.check(lib.gm_pcg(*head, int(self.mass_lumped_pc) | (2 if x0 is not None else 0), *tail), 'gm_pcg(mass)')
         self.mass_solver = 'pcg'
         if not (rr.value <= tol):
             its = it.value
-            L.check(lib.gm_bicgstab(*head, *tail), 'gm_bicgstab(mass)')
+            L.check(lib.gm_bicgstab(*head, *tail), 'gm_bicgstab(mass)')  # from x0 = 0
             it.value += its
             self.mass_solver = 'bicgstab'
         self.mass_iters = it.value

@@ -81,7 +81,10 @@ class quasi_static_coupled_solver_2d(_QuasiStaticBase):
         self._newton(current_step, total_steps, self._new, self._old, self._args(current_step, total_steps))
 
     def prepare_step(self, current_step, total_steps):
-        """reset, boundary callback, P2G and consistent-mass projection (qc2d:275-300)."""
+        """reset, boundary callback, P2G and consistent-mass projection (qc2d:275-300).
+        The projection starts from the nodal pressures the solver last held (the previous
+        load step's, from which the particle pressures were interpolated)."""
+        x0 = self._new.view(-1, self.engine.nb)[:, -1].clone() if self.warm_mass else None
         self.reset()
         eng = self.engine
         self.set_boundary_dofs(self.dofStruct.boundary_flag_array, self.n_grid_x, self.n_nodes)
@@ -91,7 +94,7 @@ class quasi_static_coupled_solver_2d(_QuasiStaticBase):
         mrhs = eng.p2g(self.x_particles, self.GIMP_lp, 1e-6, pbc, self.dy, self.p_mass, self.particle_pressure_array,
                        mass=True)
         self._sync_flags_out()
-        xp = eng.solve_mass(mrhs)
+        xp = eng.solve_mass(mrhs, x0=x0)
         eng.set_nodal_pressure(xp, self._old, self._new, require_active=False)
 
     def advance_one_step(self, current_step, total_steps):

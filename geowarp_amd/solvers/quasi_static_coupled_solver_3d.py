@@ -301,6 +301,7 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
 
         dF = 0.0
         its_total = 0
+        warm = self.warm_mass
         for k in range(n_steps):
             s0 = self.state_dict()
             self.advance_one_step(k, total_steps)
@@ -309,6 +310,7 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
             u = self._new.clone()
             s1 = self.state_dict()
             force = float(self.indentation_force.item())
+            self.warm_mass = False  # every evaluation below projects from x0 = 0, the same way
             try:
                 # dR/ds . ds/dE + dR/dE at u
                 r = []
@@ -346,6 +348,7 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
             finally:
                 self._set_E(E)
                 self.load_state_dict(s1)
+                self.warm_mass = warm
         S = abs(self.footing_incr_disp) * float(total_steps)
         self.path_force_derivative = dF
         self.E_grad_path = 2.0 * (force / S - TARGET_FORCE) / S * dF
@@ -353,7 +356,10 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
         return self.E_grad_path
 
     def prepare_step(self, current_step, total_steps):
-        """reset, boundary callback, P2G and consistent-mass projection (qc3d:397-417)."""
+        """reset, boundary callback, P2G and consistent-mass projection (qc3d:397-417).
+        The projection starts from the nodal pressures the solver last held (the previous
+        load step's, from which the particle pressures were interpolated)."""
+        x0 = self._new.view(-1, self.engine.nb)[:, -1].clone() if self.warm_mass else None
         self.reset()
         self._sync_grid()
         eng = self.engine
@@ -365,7 +371,7 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
         mrhs = eng.p2g(self.x_particles, self.GIMP_lp, 1e-7, pbc, self.dz, self.p_mass, self.particle_pressure_array,
                        mass=True)
         self._sync_flags_out()
-        xp = eng.solve_mass(mrhs)
+        xp = eng.solve_mass(mrhs, x0=x0)
         eng.set_nodal_pressure(xp, self._old, self._new, require_active=True)
 
     def advance_one_step(self, current_step, total_steps):

@@ -208,12 +208,14 @@ int gm_bicgstab(int nb, int fp32, int n_rows, const int* row_ptr, const int* col
 /* Preconditioned conjugate gradients for symmetric positive definite node block-CSR
    systems (the consistent-mass projection), same arguments, workspace and stopping rule
    as gm_bicgstab: one SpMV and two reductions per iteration, every scalar on the device.
-   lumped = 0: block-Jacobi preconditioner; 1 (nb = 1 only): the inverse row sums, which
-   for a consistent-mass matrix are the lumped nodal masses.  Stops early, with relres
-   above tol, when p.Ap is not positive (the matrix is not positive definite).  Replaces
-   the reference's Jacobi-BiCGStab of the mass projection (qc3d:410-412, qc2d:288-293). */
+   options bit 0: the inverse row sums as the preconditioner
+   (nb = 1 only; for a consistent-mass matrix the lumped nodal masses) instead of the
+   block-Jacobi inverse; bit 1: warm start from the x passed in (else x0 = 0); the test
+   stays ||r|| <= tol ||b||.  Stops early, with relres above tol, when p.Ap is not
+   positive (the matrix is not positive definite).  Replaces the reference's
+   Jacobi-BiCGStab of the mass projection (qc3d:410-412, qc2d:288-293). */
 int gm_pcg(int nb, int fp32, int n_rows, const int* row_ptr, const int* col_idx, const void* vals, const void* b,
-           void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int lumped, int* iters,
+           void* x, gm_krylov_ws* ws, double tol, int maxiter, int check_every, int options, int* iters,
            double* relres, hipStream_t stream);
 
 
