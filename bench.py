@@ -83,9 +83,10 @@ def parse():
     ap.add_argument('--forcing-max', type=float, default=None,
                     help='cap of the mixed form\'s Eisenstat-Walker inner tolerance (the solver\'s default: '
                          '1e-3 for elastic materials, 1e-4 for plastic ones)')
-    ap.add_argument('--mg-lag', type=int, default=1,
-                    help='1: keep the coarse multigrid operators of a load step\'s first Newton correction; '
-                         '0: rebuild them for every correction')
+    ap.add_argument('--mg-lag', type=int, default=None,
+                    help='0: rebuild the coarse multigrid operators for every Newton correction; 1: keep those of '
+                         'a load step\'s first correction; 2: keep them across load steps too (default: the '
+                         'solver\'s, 2 for the mixed form and 1 for FP64)')
     ap.add_argument('--budget', type=float, default=540.0,
                     help='wall-clock seconds for the whole run; the optional blocks are skipped past it')
     ap.add_argument('--no-cpu', action='store_true')
@@ -111,7 +112,8 @@ def build_solver(cells, precision, args, tol=1e-8):
                      dp_apex=args.dp_apex, shape_factor=args.shape_factor, footing_start=args.footing_start,
                      penalty_form=args.penalty_form, gravity=args.gravity)
     cfg['iteration_dict']['substeps'] = args.substeps
-    cfg['iteration_dict']['mg_lag'] = bool(args.mg_lag)
+    if args.mg_lag is not None:
+        cfg['iteration_dict']['mg_lag'] = int(args.mg_lag)
     if args.forcing_max is not None:
         cfg['iteration_dict']['forcing_max'] = args.forcing_max
     s = P.build_solver(gs, cfg, device='cuda:0')

@@ -470,7 +470,9 @@ class Engine:
             mgh = entry[0]
             full = True
             if entry[1] != self.jac_version:
-                full = not self.mg_lag or entry[2] != self.step_id or entry[4] or mgh.ublock
+                # mg_lag 1: coarse operators kept within a load step; 2: also across load steps
+                # (the coarse grids and their patterns do not change; level 0 is refreshed)
+                full = not self.mg_lag or entry[4] or mgh.ublock or (entry[2] != self.step_id and self.mg_lag < 2)
                 hv, op = self._hierarchy_matrix() if not mgh.ublock else (self.vals, None)
                 mgh.setup(_stream(), vals=hv, op_vals=op, coarse=full)
                 entry[1] = self.jac_version
