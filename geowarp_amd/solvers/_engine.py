@@ -472,16 +472,29 @@ class Engine:
             if entry[1] != self.jac_version:
                 # mg_lag 1: coarse operators kept within a load step; 2: also across load steps
                 # (the coarse grids and their patterns do not change; level 0 is refreshed)
-                full = not self.mg_lag or entry[4] or mgh.ublock or (entry[2] != self.step_id and self.mg_lag < 2)
+                # (entry[2] < 0: the coarse operators have never been built)
+                full = (not self.mg_lag or entry[4] or mgh.ublock or entry[2] < 0
+                        or (entry[2] != self.step_id and self.mg_lag < 2))
                 hv, op = self._hierarchy_matrix() if not mgh.ublock else (self.vals, None)
                 mgh.setup(_stream(), vals=hv, op_vals=op, coarse=full)
                 entry[1] = self.jac_version
                 if full:
                     entry[2], entry[4] = self.step_id, False
             V, z, w, dinv, part, dh = self.gmres_workspace(self.nb, fp32, restart)
-            its, rel = mgh.gmres(b, x, V, z, w, part, dh, restart, tol, maxiter, _stream())
+            # a solve on kept coarse operators gets at most 4x (+40) the iterations of the
+            # solve after their setup; past that they are rebuilt and the system solved again
+            cap = maxiter if full else min(maxiter, 4 * entry[3] + 40)
+            its, rel = mgh.gmres(b, x, V, z, w, part, dh, restart, tol, cap, _stream())
             if full:
                 entry[3] = its
+            elif not (rel <= tol) and cap < maxiter:
+                self.stats['mg_rebuilds'] = self.stats.get('mg_rebuilds', 0) + 1
+                hv, op = self._hierarchy_matrix()
+                mgh.setup(_stream(), vals=hv, op_vals=op, coarse=True)
+                entry[2], entry[4] = self.step_id, False
+                its2, rel = mgh.gmres(b, x, V, z, w, part, dh, restart, tol, maxiter, _stream())
+                entry[3] = its2
+                its += its2
             elif its > 3 * entry[3] + 20:
                 entry[4] = True
             return its, rel
