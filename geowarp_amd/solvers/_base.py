@@ -23,6 +23,9 @@ Extensions over the reference (all optional, defaults reproduce it):
   'gmres': block-Jacobi GMRES | 'bicgstab'), ['krylov_restart'],
   ['mg_nu'] (both), ['mg_nu_pre'], ['mg_nu_post'] (smoothing sweeps, default 1 and 2),
   ['mg_omega'] (damping, 0.4), ['mg_coarse_sweeps'], ['mg_ublock'], ['mg_min_cells']  V-cycle options
+  ['mg_lag']                      coarse operators rebuilt at every setup (0), kept within a load
+                                  step (1) or also across load steps (2: the mixed form's default
+                                  for elastic materials; 1 otherwise)
   ['mg_lowprec']                  mixed form: FP16 (row-scaled) smoother matrices (default True)
   ['mixed_fallback']              mixed form: re-solve a correction in FP64 (FP64 tangent, Jacobian
                                   and hierarchy) when the FP32 solve leaves relres > 0.1 (default True)
@@ -184,10 +187,14 @@ class _QuasiStaticBase:
         self.engine = Engine(self.dim, self.coupled, n_grid, spacing, device=self.device, precision=self.precision)
         self.engine.mixed_assembly = it.get('mixed_assembly', 'fp32')
         # coarse multigrid operators: 1 kept within a load step, 2 also across load steps (the
-        # mixed form's default: its GMRES iterations are cheap, so trading a few more of them for
-        # the Galerkin set-up pays -- 1.51 s against 1.61 s on load steps 5-24 of the 64 x 64 x 32
-        # indentation -- while the FP64 form's cost more than the set-up: 6.80 s against 6.37 s)
-        self.engine.mg_lag = int(it.get('mg_lag', 2 if self.precision == 'mixed' else 1))
+        # mixed form's default for elastic skeletons: its GMRES iterations are cheap, so trading
+        # a few more of them for the Galerkin set-up pays -- 1.51 s against 1.61 s on load steps
+        # 5-24 of the 64 x 64 x 32 indentation -- while the FP64 form's cost more than the
+        # set-up: 6.80 s against 6.37 s.  Plastic skeletons keep 1: yielding between load steps
+        # changes the tangent, and on the 256 x 1024-cell J2 bar a correction of load step 1
+        # then takes 435 iterations on step 0's coarse operators against 149 on fresh ones)
+        plastic = self.material_name in ('J2', 'Drucker-Prager', 'NorSand')
+        self.engine.mg_lag = int(it.get('mg_lag', 2 if self.precision == 'mixed' and not plastic else 1))
         self.engine.mg_options = mg_opts
         # coupled forms: start each consistent-mass projection from the last nodal pressures
         self.warm_mass = bool(it.get('warm_mass', True))

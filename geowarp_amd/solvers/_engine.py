@@ -481,9 +481,11 @@ class Engine:
                 if full:
                     entry[2], entry[4] = self.step_id, False
             V, z, w, dinv, part, dh = self.gmres_workspace(self.nb, fp32, restart)
-            # a solve on kept coarse operators gets at most 4x (+40) the iterations of the
-            # solve after their setup; past that they are rebuilt and the system solved again
-            cap = maxiter if full else min(maxiter, 4 * entry[3] + 40)
+            # a solve on kept coarse operators gets at most max(20x, 1000) the iterations of
+            # the solve after their setup (a safety net: operators that no longer fit the
+            # matrix); past that they are rebuilt and the system solved again.  Ordinary
+            # drift (more than 3x + 20) marks them for a rebuild at the next setup instead.
+            cap = maxiter if full else min(maxiter, max(20 * entry[3], 1000))
             its, rel = mgh.gmres(b, x, V, z, w, part, dh, restart, tol, cap, _stream())
             if full:
                 entry[3] = its
