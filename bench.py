@@ -20,7 +20,11 @@ smoother copies, FP64 residual); the plain FP64 form runs the same load steps
 from the same state (as many as the --budget allows) and both are reported,
 with the mixed/FP64 time ratio over the compared steps and the field errors at
 the last of them.  value = Newton iterations per second of the mixed form over
-the timed load steps; every timed step must converge, or value is null.
+the timed load steps; every timed step must converge, or value is null.  The
+mixed form solves the last correction of a load step (the one applied after the
+residual has passed the Newton test) on the previous iteration's Jacobian
+(chord_iters counts them; they stay Newton iterations of the count, each
+evaluating the residual, solving and updating as the others do).
 
 roofline: the dominant kernel of the timed form, the per-bin phase of the
 Jacobian assembly (k_jac_bin: the 27 x 27 block products of every particle bin
@@ -134,11 +138,13 @@ def run_steps(s, cfg, first, count, log=None):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         n_kry = len(s.engine.stats['krylov_iters'])
+        n_chord = s.engine.stats.get('chord_steps', 0)
         s.advance_one_step(k, cfg['total_steps'])
         torch.cuda.synchronize()
         h = s.residual_history
         rec = dict(step=k, seconds=round(time.perf_counter() - t0, 4), newton_iters=s.step_newton_iterations,
                    substeps=s.substep_count, krylov_iters=int(sum(s.engine.stats['krylov_iters'][n_kry:])),
+                   chord_iters=s.engine.stats.get('chord_steps', 0) - n_chord,
                    converged=bool(s.newton_converged),
                    relative_residual=(h[-1] / h[0]) if h and h[0] > 0 else None,
                    indentation_force=float(s.indentation_force.item()))
@@ -156,6 +162,7 @@ def summarize(recs, count):
     it = sum(r['newton_iters'] for r in recs)
     out = dict(seconds=round(sec, 4), load_steps=len(recs), all_converged=ok, newton_iters=it,
                krylov_iters=sum(r['krylov_iters'] for r in recs), substeps=sum(r['substeps'] for r in recs),
+               chord_iters=sum(r['chord_iters'] for r in recs),
                newton_per_s=round(it / sec, 4) if sec > 0 else None,
                ms_per_load_step=round(1e3 * sec / max(len(recs), 1), 3))
     if not ok and recs:

@@ -27,6 +27,9 @@ Extensions over the reference (all optional, defaults reproduce it):
                                   step (1) or also across load steps (2: the mixed form's default
                                   for elastic materials; 1 otherwise)
   ['mg_lowprec']                  mixed form: FP16 (row-scaled) smoother matrices (default True)
+  ['final_chord']                 solve the last correction (applied after the residual passes the
+                                  Newton test) on the previous iteration's Jacobian (default: the
+                                  mixed form with an elastic material)
   ['mixed_fallback']              mixed form: re-solve a correction in FP64 (FP64 tangent, Jacobian
                                   and hierarchy) when the FP32 solve leaves relres > 0.1 (default True)
   ['mixed_assembly']              mixed form: 'fp32' tangent and Jacobian (default) or 'fp64'
@@ -167,6 +170,11 @@ class _QuasiStaticBase:
         self.final_forcing = it.get('final_forcing', 1e-3)
         # absolute floor of the Newton test (None: the tolerance alone)
         self.abs_floor = it.get('abs_floor')
+        # the last correction (its residual already passes the Newton test) solved on the
+        # previous iteration's Jacobian, a chord step: the mixed form's default for elastic
+        # materials, whose Jacobian barely moves over the last Newton correction
+        self.final_chord = bool(it.get('final_chord', self.precision == 'mixed' and self.material_name not in
+                                       ('J2', 'Drucker-Prager', 'NorSand')))
         if self.material_name not in MAT_KIND:
             raise ValueError("Unimplemented material model: %s" % self.material_name)
         self.youngs_modulus = m['youngs_modulus']
@@ -299,6 +307,7 @@ class _QuasiStaticBase:
         eta = None
         nr_prev = None
         self._ls_ready = None  # norm of an accepted line-search trial whose stage had the tangent
+        jac_fresh = False  # the engine holds this load step's last Newton Jacobian (FP32 in the mixed form)
         for it in range(self.n_iter):
             if tm is not None:
                 tm.mark('start')
@@ -315,15 +324,24 @@ class _QuasiStaticBase:
                 after_residual()
             if tm is not None:
                 tm.mark('node_residual')
-            eng.jacobian(x, lp)
-            if self.autotune and eng.stats.get('jacobian_tune') is None and eng.step_id >= 2:
-                # tuned from the second load step on: the undeformed first step has a sparser block pattern
-                eng.autotune_jacobian()
-            if self.autotune and eng.stats.get('spmv_tune') is None:
-                eng.autotune_spmv()
+            nr = reuse
+            if nr is None and r0 is not None and self.final_chord:
+                nr = math.sqrt(max(eng.scal[0].item(), 0.0))  # decides whether the Jacobian is assembled
+            if self.final_chord and jac_fresh and nr is not None and r0 is not None and self._passes(nr, r0):
+                # the correction applied after the passing residual: on the last Jacobian
+                eng.stats['chord_steps'] = eng.stats.get('chord_steps', 0) + 1
+            else:
+                eng.jacobian(x, lp)
+                jac_fresh = True
+                if self.autotune and eng.stats.get('jacobian_tune') is None and eng.step_id >= 2:
+                    # tuned from the second load step on: the undeformed first step has a sparser block pattern
+                    eng.autotune_jacobian()
+                if self.autotune and eng.stats.get('spmv_tune') is None:
+                    eng.autotune_spmv()
             if tm is not None:
                 tm.mark('node_jacobian')
-            nr = reuse if reuse is not None else math.sqrt(max(eng.scal[0].item(), 0.0))
+            if nr is None:
+                nr = math.sqrt(max(eng.scal[0].item(), 0.0))
             tol_lin = self.linear_tol
             if self.forcing == 'ew':
                 if nr_prev is None:
@@ -346,6 +364,7 @@ class _QuasiStaticBase:
                 # the Jacobian in FP64 at the same iterate and solve this correction in FP64
                 prev = eng.mixed_assembly
                 eng.mixed_assembly = 'fp64'
+                jac_fresh = False  # the matrices now hold the FP64 assembly
                 try:
                     eng.stage(args, want_tangent=True)
                     eng.jacobian(x, lp)
