@@ -468,7 +468,8 @@ class Engine:
         if method == 'mg':
             entry = self.multigrid(fp32)
             mgh = entry[0]
-            full = True
+            full = False  # coarse operators built for this solve (not for a refinement sweep or a
+            # later solve on the same Jacobian, which reuse the set-up as it stands)
             if entry[1] != self.jac_version:
                 # mg_lag 1: coarse operators kept within a load step; 2: also across load steps
                 # (the coarse grids and their patterns do not change; level 0 is refreshed)
