@@ -653,10 +653,12 @@ extern "C" int gm_bicgstab(int nb, int fp32, int n_rows, const int* row_ptr, con
 // Conjugate gradients (Hestenes & Stiefel 1952) with a diagonal-block preconditioner for
 // symmetric positive definite systems (the consistent-mass projection): per iteration
 // the SpMV q = A p with fused p.q partials, one workgroup summing them into alpha, the
-// update x += alpha p, r -= alpha q, z = M^-1 r with fused r.z and r.r partials, one
-// workgroup summing them into beta, and p = z + beta p; every scalar stays on the
-// device and the host reads ||r|| every `check_every` iterations.
+// update x += alpha p, r -= alpha q, z = M^-1 r with fused r.z and r.r partials, and
+// p = z + beta p, every workgroup of which sums those partials itself (in one fixed
+// order) into beta; every scalar stays on the device and the host reads ||r|| every
+// `check_every` iterations.  C_RZN holds the newest r.z, C_RZ the one alpha last used.
 enum { C_RZ = 0, C_RZN, C_PQ, C_RR, C_BB, C_ALPHA, C_BETA, C_FLAG, C_NSC };
+#define PCG_WIDE 1024
 
 // x = 0 (cold start) or kept (warm start, ax = A x), r = b - ax, z = Dinv r, p = z;
 // per-workgroup partials of r.z and r.r
@@ -725,29 +727,41 @@ __device__ void sum_pairs(const double* part, int n, double* s0, double* s1) {
   *s1 = b[0];
 }
 
-// p.q from the SpMV's partials; alpha = (r.z) / (p.q); a non-positive or non-finite
-// p.q stops the iteration (flag, alpha = 0)
-__global__ void __launch_bounds__(RED_T) k_pcg_alpha(const double* part, int n, double* sc) {
-  double pq, unused;
-  sum_pairs(part, n, &pq, &unused);
-  if (threadIdx.x == 0) {
-    sc[C_PQ] = pq;
-    const bool ok = pq > 0.0 && pq < 1e308 && sc[C_FLAG] == 0.0;
-    sc[C_ALPHA] = ok ? sc[C_RZ] / pq : 0.0;
-    if (!ok) sc[C_FLAG] = 1.0;
+// the first of every partial pair, summed by a workgroup of PCG_WIDE threads: four
+// independent accumulators per thread (loads in flight), then a tree in LDS; one fixed
+// order for a given n
+__device__ double sum_firsts_wide(const double* part, int n) {
+  __shared__ double a[PCG_WIDE];
+  double u0 = 0.0, u1 = 0.0, u2 = 0.0, u3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * PCG_WIDE < n; i += 4 * PCG_WIDE) {
+    u0 += part[2 * i];
+    u1 += part[2 * (i + PCG_WIDE)];
+    u2 += part[2 * (i + 2 * PCG_WIDE)];
+    u3 += part[2 * (i + 3 * PCG_WIDE)];
   }
+  for (; i < n; i += PCG_WIDE) u0 += part[2 * i];
+  a[threadIdx.x] = (u0 + u1) + (u2 + u3);
+  __syncthreads();
+  for (int o = PCG_WIDE / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) a[threadIdx.x] += a[threadIdx.x + o];
+    __syncthreads();
+  }
+  return a[0];
 }
 
-// r.z and r.r from the update's partials; beta = (r.z)_new / (r.z)
-__global__ void __launch_bounds__(RED_T) k_pcg_beta(const double* part, int n, double* sc) {
-  double rzn, rr;
-  sum_pairs(part, n, &rzn, &rr);
+// p.q from the SpMV's partials; alpha = (r.z) / (p.q) with r.z the newest (C_RZN, kept
+// in C_RZ for the next beta); a non-positive or non-finite p.q stops the iteration
+// (flag, alpha = 0)
+__global__ void __launch_bounds__(PCG_WIDE) k_pcg_alpha(const double* part, int n, double* sc) {
+  const double pq = sum_firsts_wide(part, n);
   if (threadIdx.x == 0) {
-    const double rz = sc[C_RZ];
-    sc[C_RZN] = rzn;
-    sc[C_RR] = rr;
-    sc[C_BETA] = (rz != 0.0) ? rzn / rz : 0.0;
-    sc[C_RZ] = rzn;
+    const double rz = sc[C_RZN];
+    sc[C_PQ] = pq;
+    sc[C_RZ] = rz;
+    const bool ok = pq > 0.0 && pq < 1e308 && sc[C_FLAG] == 0.0;
+    sc[C_ALPHA] = ok ? rz / pq : 0.0;
+    if (!ok) sc[C_FLAG] = 1.0;
   }
 }
 
@@ -789,9 +803,20 @@ __global__ void k_pcg_xr(int n_rows, const double* sc, TV* x, TV* r, const TV* p
   }
 }
 
+// r.z and r.r from the update's nparts partials (every workgroup sums them in the same
+// order), beta = (r.z)_new / (r.z), p = z + beta p; workgroup 0 stores r.z, r.r and beta
 template <typename TV>
-__global__ void k_pcg_p(long n, const double* sc, const TV* z, TV* p) {
-  const double beta = sc[C_BETA];
+__global__ void __launch_bounds__(RED_T) k_pcg_p(long n, const double* part, int nparts, double* sc, const TV* z,
+                                                 TV* p) {
+  double rzn, rr;
+  sum_pairs(part, nparts, &rzn, &rr);
+  const double rz = sc[C_RZ];
+  const double beta = (rz != 0.0) ? rzn / rz : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[C_RZN] = rzn;
+    sc[C_RR] = rr;
+    sc[C_BETA] = beta;
+  }
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     p[i] = (TV)((double)z[i] + beta * (double)p[i]);
 }
@@ -830,7 +855,7 @@ static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, co
   k_sum_parts<<<1, RED_T, 0, st>>>(part, dot_blocks(n), sc + C_BB, nullptr);
   if (warm) launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, x, q, nullptr, nullptr, nullptr, st);
   k_pcg_start<NB, TM, TV><<<db, RED_T, 0, st>>>(n_rows, b, warm ? q : nullptr, dinv, x, r, z, p, part);
-  k_sum_parts<<<1, RED_T, 0, st>>>(part, db, sc + C_RZ, sc + C_RR);
+  k_sum_parts<<<1, RED_T, 0, st>>>(part, db, sc + C_RZN, sc + C_RR);
   GM_LAUNCH_CHECK();
   double hs[C_NSC];
   GM_HIP(hipMemcpyAsync(hs, sc, sizeof(double) * C_NSC, hipMemcpyDeviceToHost, st));
@@ -853,10 +878,9 @@ static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, co
     const int nrun = std::min(check_every, maxiter - it);
     for (int k = 0; k < nrun; ++k) {
       launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, p, q, p, nullptr, part, st);
-      k_pcg_alpha<<<1, RED_T, 0, st>>>(part, sb, sc);
+      k_pcg_alpha<<<1, PCG_WIDE, 0, st>>>(part, sb, sc);
       k_pcg_xr<NB, TM, TV><<<db, RED_T, 0, st>>>(n_rows, sc, x, r, p, q, dinv, z, part);
-      k_pcg_beta<<<1, RED_T, 0, st>>>(part, db, sc);
-      k_pcg_p<TV><<<gm_blocks(n, 256), 256, 0, st>>>(n, sc, z, p);
+      k_pcg_p<TV><<<gm_blocks(n, RED_T), RED_T, 0, st>>>(n, part, db, sc, z, p);
     }
     GM_LAUNCH_CHECK();
     it += nrun;
