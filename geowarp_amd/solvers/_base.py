@@ -318,13 +318,16 @@ class _QuasiStaticBase:
         tm = eng.timers
         eta = None
         nr_prev = None
-        self._ls_ready = None  # norm of an accepted line-search trial whose stage had the tangent
+        self._ls_ready = None  # norm of an accepted line-search trial (full step)
+        self._ls_tangent = True  # whether that trial's stage computed the tangent
         jac_fresh = False  # the engine holds this load step's last Newton Jacobian (FP32 in the mixed form)
         for it in range(self.n_iter):
             if tm is not None:
                 tm.mark('start')
             reuse = self._ls_ready
+            has_tangent = reuse is None or self._ls_tangent
             self._ls_ready = None
+            self._aims_to_pass = False
             if reuse is None:
                 eng.stage(args, want_tangent=True)
                 if tm is not None:
@@ -343,6 +346,9 @@ class _QuasiStaticBase:
                 # the correction applied after the passing residual: on the last Jacobian
                 eng.stats['chord_steps'] = eng.stats.get('chord_steps', 0) + 1
             else:
+                if not has_tangent:
+                    # the line search skipped the tangent, expecting this residual to pass
+                    eng.stage(args, want_tangent=True)
                 eng.jacobian(x, lp)
                 jac_fresh = True
                 if self.autotune and eng.stats.get('jacobian_tune') is None and eng.step_id >= 2:
@@ -433,10 +439,13 @@ class _QuasiStaticBase:
         a = 1.0
         finite = False
         accepted = False
+        # the full step's tangent serves the next Newton Jacobian; none is assembled after a
+        # correction solved to pass the Newton test when the last correction is a chord step
+        full_tangent = not (self.final_chord and getattr(self, '_aims_to_pass', False))
         for _ in range(max_halvings + 1):
             u_new.copy_(base)
             eng.axpy(a, eng.delta, u_new)
-            eng.stage(args, want_tangent=(a == 1.0))
+            eng.stage(args, want_tangent=(a == 1.0 and full_tangent))
             eng.residual(self.x_particles, self.GIMP_lp)
             eng.norm2(eng.rhs, 0)
             trial = math.sqrt(max(eng.scal[0].item(), 0.0))
@@ -451,6 +460,7 @@ class _QuasiStaticBase:
             raise L.GMError('line search: no finite residual along the Newton correction (down to step %g)' % a)
         if accepted and a == 1.0:
             self._ls_ready = trial
+            self._ls_tangent = full_tangent
         self.line_search_steps.append(a)
 
     def _ew_forcing(self, nr, nr_prev, eta_prev, r0):
@@ -467,12 +477,14 @@ class _QuasiStaticBase:
                 eta = max(eta, sg)
         target = self.tol * r0 if self.tol_mode == 'relative' else self.tol
         cap = getattr(self, '_eta_cap', self.eta_max)
+        self._aims_to_pass = False
         if nr > 0:
             eta = max(eta, 0.5 * target / nr)
             if nr * cap < target:
                 # the residual is within 1/cap of the target: solve to half the target, so that
                 # the next residual can pass the test
                 eta = min(eta, 0.5 * target / nr)
+                self._aims_to_pass = 0.5 * target / nr >= self.linear_tol
         return min(max(eta, self.linear_tol), cap)
 
     # ------------------------------------------------------------------ checkpoint / restart
