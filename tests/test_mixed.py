@@ -72,3 +72,55 @@ def test_mixed_form_fp64_fallback_matches_oracle(name):
     s = _run_against_fixture(name, dict(ir_steps=1), cap)
     print(name, 'fallbacks', s.engine.stats.get('fp64_fallbacks'))
     assert s.engine.stats.get('fp64_fallbacks', 0) > 0
+
+
+@pytest.mark.gpu
+def test_chord_step_and_skipped_trial_tangent():
+    """Elastic skeletons in the mixed form solve a load step's last correction on the
+    previous Jacobian (a chord step) and skip the particle tangent in the line-search
+    trial of the correction solved to pass the Newton test.  Forcing every trial to skip
+    it makes the Newton iteration compute the tangent again wherever a Jacobian is
+    assembled, which must give the same bits; both runs meet the oracle bar of the
+    indentation fixture (line search on)."""
+    import torch
+    import geowarp_amd.solvers as gs
+    g = np.load(os.path.join(GOLD, 'indent_nh.npz'))
+    total = int(g['total_steps'][0])
+    out = {}
+    for forced in (False, True):
+        cfg = H.MINI['indent_nh']()
+        cfg['iteration_dict'] = dict(cfg['iteration_dict'], precision='mixed', line_search=True)
+        s = H.make(gs, cfg)
+        assert s.final_chord
+        calls = {'tangent': 0, 'plain': 0}
+        stage = s.engine.stage
+
+        def counted(args, want_tangent=True, _stage=stage, _calls=calls):
+            _calls['tangent' if want_tangent else 'plain'] += 1
+            return _stage(args, want_tangent=want_tangent)
+        s.engine.stage = counted
+        if forced:
+            ew = s._ew_forcing
+
+            def always_skip(*a, _ew=ew, _s=s):
+                eta = _ew(*a)
+                _s._aims_to_pass = True
+                return eta
+            s._ew_forcing = always_skip
+        x0 = s.x0.cpu().numpy()
+        for k in range(int(g['n_steps'][0])):
+            s.advance_one_step(k, total)
+            assert s.newton_converged
+            f = H.fields(s)
+            for key in ('x', 'sigma', 'p'):
+                ref = g['s%d_%s' % (k, key)]
+                e = H.rel_err(f[key] - x0, ref - x0) if key == 'x' else H.rel_err(f[key], ref)
+                assert e < 1e-6, (forced, k, key, e)
+        torch.cuda.synchronize()
+        out[forced] = (s.x_particles.clone(), s.particle_Cauchy_stress_array.clone(), s.particle_pressure_array.clone(),
+                       dict(calls), s.engine.stats.get('chord_steps', 0))
+        print('forced skip' if forced else 'default', 'stage calls', calls, 'chord steps', out[forced][4])
+    (xa, sa, pa, ca, na), (xb, sb, pb, cb, nb) = out[False], out[True]
+    assert na > 0 and nb == na
+    assert cb['plain'] > 0 and cb['plain'] >= ca['plain'] and cb['tangent'] >= ca['tangent']
+    assert torch.equal(xa, xb) and torch.equal(sa, sb) and torch.equal(pa, pb)
