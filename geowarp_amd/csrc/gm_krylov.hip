@@ -27,7 +27,9 @@
 // ------------------------------------------------------------------ SpMV
 template <int NB, int G, typename TM, typename TX, typename TY, int T = SPMV_T>
 __global__ void __launch_bounds__(T) k_spmv(int n_rows, const int* row_ptr, const int* col_idx, const TM* vals,
-                                            const TX* x, TY* y, const TY* w1, const TY* w2, double* part) {
+                                            const TX* x, TY* y, const TY* w1, const TY* w2, double* part,
+                                            const double* gate) {
+  if (gate && *gate != 0.0) return;  // a gated launch after its solve has converged
   const int rows_per_blk = T / G;
   int lr = threadIdx.x / G;
   int lane = threadIdx.x % G;
@@ -117,24 +119,26 @@ static constexpr int spmv_lanes(int nb) { return nb >= 4 ? 16 : (nb == 3 ? 8 : (
 
 template <int NB, typename TM, typename TX, typename TY>
 static void launch_spmv(int n_rows, const int* rp, const int* ci, const TM* v, const TX* x, TY* y, const TY* w1,
-                        const TY* w2, double* part, hipStream_t s, int threads = 0) {
+                        const TY* w2, double* part, hipStream_t s, int threads = 0,
+                        const double* gate = nullptr) {
   const int G = spmv_lanes(NB);
   int t = part ? SPMV_T : (threads ? threads : g_spmv_threads[NB][sizeof(TM) == 4 ? 1 : 0]);
   switch (t) {
     case 64:
-      k_spmv<NB, G, TM, TX, TY, 64><<<gm_blocks(n_rows, 64 / G), 64, 0, s>>>(n_rows, rp, ci, v, x, y, w1, w2, part);
+      k_spmv<NB, G, TM, TX, TY, 64><<<gm_blocks(n_rows, 64 / G), 64, 0, s>>>(n_rows, rp, ci, v, x, y, w1, w2, part,
+                                                                               gate);
       break;
     case 128:
       k_spmv<NB, G, TM, TX, TY, 128><<<gm_blocks(n_rows, 128 / G), 128, 0, s>>>(n_rows, rp, ci, v, x, y, w1, w2,
-                                                                                 part);
+                                                                                 part, gate);
       break;
     case 512:
       k_spmv<NB, G, TM, TX, TY, 512><<<gm_blocks(n_rows, 512 / G), 512, 0, s>>>(n_rows, rp, ci, v, x, y, w1, w2,
-                                                                                 part);
+                                                                                 part, gate);
       break;
     default:
       k_spmv<NB, G, TM, TX, TY, 256><<<gm_blocks(n_rows, 256 / G), 256, 0, s>>>(n_rows, rp, ci, v, x, y, w1, w2,
-                                                                                 part);
+                                                                                 part, gate);
   }
 }
 
@@ -697,7 +701,12 @@ extern "C" int gm_bicgstab(int nb, int fp32, int n_rows, const int* row_ptr, con
 // p = z + beta p, every workgroup of which sums those partials itself (in one fixed
 // order) into beta; every scalar stays on the device and the host reads ||r|| every
 // `check_every` iterations.  C_RZN holds the newest r.z, C_RZ the one alpha last used.
-enum { C_RZ = 0, C_RZN, C_PQ, C_RR, C_BB, C_ALPHA, C_BETA, C_FLAG, C_NSC };
+// The iteration stops on the device: the p update's workgroup 0 counts the iteration
+// (C_ITS) and sets C_DONE once ||r|| meets the tolerance; the SpMV, alpha and the x, r
+// update of later iterations then return at once (they read C_DONE, written by an earlier
+// launch), and alpha copies C_DONE into C_GATE, which gates the p update, so that no
+// launch reads a flag it writes itself.
+enum { C_RZ = 0, C_RZN, C_PQ, C_RR, C_BB, C_ALPHA, C_BETA, C_FLAG, C_DONE, C_GATE, C_ITS, C_NSC };
 #define PCG_WIDE 1024
 
 // x = 0 (cold start) or kept (warm start, ax = A x), r = b - ax, z = Dinv r, p = z;
@@ -743,7 +752,12 @@ __global__ void k_pcg_start(int n_rows, const TV* b, const TV* ax, const TM* din
   }
 }
 
-__global__ void k_pcg_init_sc(double* sc) { sc[C_FLAG] = 0.0; }
+__global__ void k_pcg_init_sc(double* sc) {
+  sc[C_FLAG] = 0.0;
+  sc[C_DONE] = 0.0;
+  sc[C_GATE] = 0.0;
+  sc[C_ITS] = 0.0;
+}
 
 // one workgroup: the sums of n partial pairs in the order of k_sum_parts
 __device__ void sum_pairs(const double* part, int n, double* s0, double* s1) {
@@ -794,6 +808,10 @@ __device__ double sum_firsts_wide(const double* part, int n) {
 // in C_RZ for the next beta); a non-positive or non-finite p.q stops the iteration
 // (flag, alpha = 0)
 __global__ void __launch_bounds__(PCG_WIDE) k_pcg_alpha(const double* part, int n, double* sc) {
+  if (sc[C_DONE] != 0.0) {
+    if (threadIdx.x == 0) sc[C_GATE] = 1.0;
+    return;
+  }
   const double pq = sum_firsts_wide(part, n);
   if (threadIdx.x == 0) {
     const double rz = sc[C_RZN];
@@ -809,6 +827,7 @@ __global__ void __launch_bounds__(PCG_WIDE) k_pcg_alpha(const double* part, int 
 template <int NB, typename TM, typename TV>
 __global__ void k_pcg_xr(int n_rows, const double* sc, TV* x, TV* r, const TV* p, const TV* q, const TM* dinv,
                          TV* z, double* part) {
+  if (sc[C_DONE] != 0.0) return;
   __shared__ double a[RED_T], c[RED_T];
   const double alpha = sc[C_ALPHA];
   double s0 = 0.0, s1 = 0.0;
@@ -844,10 +863,12 @@ __global__ void k_pcg_xr(int n_rows, const double* sc, TV* x, TV* r, const TV* p
 }
 
 // r.z and r.r from the update's nparts partials (every workgroup sums them in the same
-// order), beta = (r.z)_new / (r.z), p = z + beta p; workgroup 0 stores r.z, r.r and beta
+// order), beta = (r.z)_new / (r.z), p = z + beta p; workgroup 0 stores r.z, r.r and beta,
+// counts the iteration and sets C_DONE once r.r <= thr2 (the squared absolute tolerance)
 template <typename TV>
 __global__ void __launch_bounds__(RED_T) k_pcg_p(long n, const double* part, int nparts, double* sc, const TV* z,
-                                                 TV* p) {
+                                                 TV* p, double thr2) {
+  if (sc[C_GATE] != 0.0) return;
   double rzn, rr;
   sum_pairs(part, nparts, &rzn, &rr);
   const double rz = sc[C_RZ];
@@ -856,6 +877,8 @@ __global__ void __launch_bounds__(RED_T) k_pcg_p(long n, const double* part, int
     sc[C_RZN] = rzn;
     sc[C_RR] = rr;
     sc[C_BETA] = beta;
+    sc[C_ITS] += 1.0;
+    if (rr <= thr2) sc[C_DONE] = 1.0;
   }
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     p[i] = (TV)((double)z[i] + beta * (double)p[i]);
@@ -913,19 +936,20 @@ static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, co
     return 0;
   }
   double rel = sqrt(fmax(hs[C_RR], 0.0)) / bnorm;
+  const double thr2 = (tol * bnorm) * (tol * bnorm);
   int it = 0;
   while (it < maxiter && rel > tol) {
     const int nrun = std::min(check_every, maxiter - it);
     for (int k = 0; k < nrun; ++k) {
-      launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, p, q, p, nullptr, part, st);
+      launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, p, q, p, nullptr, part, st, 0, sc + C_DONE);
       k_pcg_alpha<<<1, PCG_WIDE, 0, st>>>(part, sb, sc);
       k_pcg_xr<NB, TM, TV><<<db, RED_T, 0, st>>>(n_rows, sc, x, r, p, q, dinv, z, part);
-      k_pcg_p<TV><<<gm_blocks(n, RED_T), RED_T, 0, st>>>(n, part, db, sc, z, p);
+      k_pcg_p<TV><<<gm_blocks(n, RED_T), RED_T, 0, st>>>(n, part, db, sc, z, p, thr2);
     }
     GM_LAUNCH_CHECK();
-    it += nrun;
     GM_HIP(hipMemcpyAsync(hs, sc, sizeof(double) * C_NSC, hipMemcpyDeviceToHost, st));
     GM_HIP(hipStreamSynchronize(st));
+    it = (int)hs[C_ITS];
     rel = sqrt(fmax(hs[C_RR], 0.0)) / bnorm;
     if (!(hs[C_RR] == hs[C_RR])) {
       gm::set_error("pcg: residual became NaN after %d iterations", it);
@@ -934,6 +958,7 @@ static int pcg_impl(int n_rows, const int* rp, const int* ci, const TM* vals, co
       return -5;
     }
     if (hs[C_FLAG] != 0.0) break;  // p.Ap <= 0: the caller sees rel
+    if (hs[C_DONE] != 0.0) break;  // converged on the device (rel may round to just above tol)
   }
   *iters_out = it;
   *relres_out = rel;

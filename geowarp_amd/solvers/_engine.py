@@ -260,13 +260,15 @@ class Engine:
         it = L.c_int(0)
         rr = L.c_double(0.0)
         head = (1, 0, self.n_nodes, L.ptr(self.row_ptr), L.ptr(self.col_idx), L.ptr(self.mvals), L.ptr(mrhs),
-                L.ptr(xp), L.ctypes.byref(ws), tol, maxiter, 4)
+                L.ptr(xp), L.ctypes.byref(ws), tol, maxiter)
         tail = (L.ctypes.byref(it), L.ctypes.byref(rr), _stream())
-        L.check(lib.gm_pcg(*head, int(self.mass_lumped_pc) | (2 if x0 is not None else 0), *tail), 'gm_pcg(mass)')
+        # CG stops on the device; the host reads its state every 16 iterations
+        L.check(lib.gm_pcg(*head, 16, int(self.mass_lumped_pc) | (2 if x0 is not None else 0), *tail),
+                'gm_pcg(mass)')
         self.mass_solver = 'pcg'
         if not (rr.value <= tol):
             its = it.value
-            L.check(lib.gm_bicgstab(*head, *tail), 'gm_bicgstab(mass)')  # from x0 = 0
+            L.check(lib.gm_bicgstab(*head, 4, *tail), 'gm_bicgstab(mass)')  # from x0 = 0
             it.value += its
             self.mass_solver = 'bicgstab'
         self.mass_iters = it.value

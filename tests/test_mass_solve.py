@@ -2,7 +2,8 @@
 (gm_bicgstab) and against SciPy's direct solve of the same matrix, on the 3D u-p
 indentation after one load step: cold and warm starts, diagonal and lumped (row-sum)
 preconditioners.  Every solution satisfies ||M x - b|| <= 1e-13 ||b|| in the solvers'
-own stopping rule and agrees with the direct solve to 1e-11."""
+own stopping rule and agrees with the direct solve to 1e-11.  CG stops on the device, so
+the interval at which the host reads its state leaves the iterate's bits unchanged."""
 import numpy as np
 import pytest
 
@@ -65,3 +66,17 @@ def test_pcg_mass_projection_matches_bicgstab_and_direct_solve():
               'vs direct %.2e' % err)
         assert rr.value <= 1e-13 and true_rel <= 2e-13 and err <= 1e-11, name
     assert out['pcg'][0] < 2 * out['bicgstab'][0] + 8  # one SpMV per CG step, two per BiCGStab step
+    # CG stops on the device at the first iterate that meets the tolerance: how often the
+    # host reads its state changes neither that iterate nor the iteration count
+    res = []
+    for every in (1, 16, 64):
+        x = p_prev.clone()
+        it, rr = L.c_int(0), L.c_double(0.0)
+        head = (1, 0, n, L.ptr(eng.row_ptr), L.ptr(eng.col_idx), L.ptr(eng.mvals), L.ptr(mrhs), L.ptr(x),
+                L.ctypes.byref(ws), 1e-13, 5000, every)
+        L.check(lib.gm_pcg(*head, 2, L.ctypes.byref(it), L.ctypes.byref(rr), _stream()), 'pcg every %d' % every)
+        res.append((x.clone(), it.value, rr.value))
+        print('check every', every, 'iterations', it.value, 'relres', rr.value)
+    assert res[0][1] == out['pcg warm'][0]
+    for xr, its, rel in res[1:]:
+        assert its == res[0][1] and rel == res[0][2] and torch.equal(xr, res[0][0])
