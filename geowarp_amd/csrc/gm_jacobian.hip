@@ -711,6 +711,112 @@ extern "C" int gm_mass_binned(const gm_grid* g, const gm_bins* b, const gm_patte
   return 0;
 }
 
+// ================================================================== P2G (3D)
+// The activation flags, pressure-boundary flags and consistent-mass right-hand side of
+// P2G_coupled_3d (mpm_utils.py:260-386) in two passes: per bin, each of the 27 nodes of
+// its 3x3x3 box accumulates its particles' terms in bin order (the right-hand side
+// w_I m_p p_p sum_J w_J, activation when w_I > threshold inside the particle's support,
+// pressure boundary over all 27 nodes when the node lies at or above a flagged
+// particle); per node, a fixed-order pass over the <= 27 bins holding it.
+__global__ void __launch_bounds__(64) k_p2g_bin(gm_grid g, const double* x_s, const double* lp_s,
+                                                const int* cell_start, const int* sorted, const uint8_t* ext_s,
+                                                double threshold, const uint8_t* pbc_flag, double node_h_last,
+                                                double p_mass, const double* p_part, double* cb, uint8_t* cf) {
+  constexpr int PB = 16;
+  __shared__ double W[PB][3][3];
+  const int b = blockIdx.x;
+  const int ks = cell_start[b], ke = cell_start[b + 1];
+  if (ks == ke) return;  // uniform across the workgroup
+  const int t = threadIdx.x;
+  const int ox = t % 3, oy = (t / 3) % 3, oz = t / 9;
+  const int nxy = (g.nx + 1) * (g.ny + 1);
+  const double node_z = (double)(b / nxy + oz) * node_h_last;
+  double rhs = 0.0;
+  bool a_on = false, pb = false;
+  for (int p0 = ks; p0 < ke; p0 += PB) {
+    const int np = min(PB, ke - p0);
+    __syncthreads();
+    if (t < PB * 3) {
+      const int p = t / 3, a = t % 3;
+      if (p < np) {
+        const Axis ax = gimp_axis(x_s[(long)(p0 + p) * 3 + a], lp_s[(long)(p0 + p) * 3 + a], g.h[a], g.inv_h[a]);
+        W[p][a][0] = ax.w[0];
+        W[p][a][1] = ax.w[1];
+        W[p][a][2] = ax.w[2];
+      }
+    }
+    __syncthreads();
+    if (t < 27) {
+      for (int p = 0; p < np; ++p) {
+        const int k = p0 + p;
+        const int pid = sorted[k];
+        if (pbc_flag && pbc_flag[pid] && node_z >= x_s[(long)k * 3 + 2]) pb = true;
+        const uint8_t ex = ext_s[k];
+        const bool in = ox < ((ex & 1) ? 3 : 2) && oy < ((ex & 2) ? 3 : 2) && oz < ((ex & 4) ? 3 : 2);
+        if (!in) continue;
+        const double w = W[p][0][ox] * W[p][1][oy] * W[p][2][oz];
+        if (threshold < 0.0 || w > threshold) a_on = true;
+        if (p_part) {
+          double wsum = 1.0;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) wsum *= W[p][a][0] + W[p][a][1] + W[p][a][2];
+          rhs += w * wsum * p_mass * p_part[pid];
+        }
+      }
+    }
+  }
+  if (t < 27) {
+    cb[(long)b * 27 + t] = rhs;
+    cf[(long)b * 27 + t] = (uint8_t)((a_on ? 1 : 0) | (pb ? 2 : 0));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_p2g_gather(gm_grid g, const int* cell_start, const double* cb,
+                                                    const uint8_t* cf, uint8_t* act, uint8_t* bflag, double* mrhs) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= g.n_nodes) return;
+  const int nx1 = g.nx + 1, nxy = (g.nx + 1) * (g.ny + 1);
+  const int ci[3] = {I % nx1, (I / nx1) % (g.ny + 1), I / nxy};
+  double s = 0.0;
+  int flags = 0;
+  for (int o = 0; o < 27; ++o) {
+    const int bc[3] = {ci[0] - o % 3, ci[1] - (o / 3) % 3, ci[2] - o / 9};
+    if (bc[0] < 0 || bc[1] < 0 || bc[2] < 0) continue;
+    const int bn = bc[0] + bc[1] * nx1 + bc[2] * nxy;
+    if (cell_start[bn + 1] == cell_start[bn]) continue;
+    s += cb[(long)bn * 27 + o];
+    flags |= cf[(long)bn * 27 + o];
+  }
+  if (flags & 1)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) act[I * 4 + k] = 1;
+  if ((flags & 2) && bflag) bflag[I * 4 + 3] = 1;
+  if (mrhs) mrhs[I] = s;
+}
+
+extern "C" size_t gm_p2g_work_bytes(const gm_grid* g) {
+  return g ? (size_t)g->n_nodes * 27 * (sizeof(double) + 1) : 0;
+}
+
+extern "C" int gm_p2g_binned(const gm_grid* g, const gm_bins* b, double threshold, const uint8_t* pbc_flag,
+                             double node_h_last, double p_mass, const double* p_part, uint8_t* act, uint8_t* bflag,
+                             double* mrhs, void* work, size_t work_bytes, hipStream_t s) {
+  GM_REQUIRE(g && b && act && work, "gm_p2g_binned: null argument");
+  GM_REQUIRE(g->dim == 3 && g->nb == 4, "gm_p2g_binned: the 3D u-p form only");
+  GM_REQUIRE(!mrhs || p_part, "gm_p2g_binned: the right-hand side needs the particle pressures");
+  GM_REQUIRE(work_bytes >= gm_p2g_work_bytes(g), "gm_p2g_binned: workspace smaller than gm_p2g_work_bytes");
+  const int n = g->n_nodes;
+  if (n == 0) return 0;
+  double* cb = (double*)work;
+  uint8_t* cf = (uint8_t*)(cb + (size_t)n * 27);
+  k_p2g_bin<<<n, 64, 0, s>>>(*g, b->x_s, b->lp_s, b->cell_start, b->sorted, b->ext_s, threshold, pbc_flag,
+                             node_h_last, p_mass, mrhs ? p_part : nullptr, cb, cf);
+  GM_LAUNCH_CHECK();
+  k_p2g_gather<<<gm_blocks(n, 256), 256, 0, s>>>(*g, b->cell_start, cb, cf, act, bflag, mrhs);
+  GM_LAUNCH_CHECK();
+  return 0;
+}
+
 extern "C" size_t gm_node_jacobian_work_bytes(const gm_grid* g, int tangent_fp32) {
   if (!g) return 0;
   return (size_t)g->n_nodes * (size_t)CB_BIN * (tangent_fp32 ? sizeof(float) : sizeof(double));

@@ -110,6 +110,9 @@ class Engine:
         # then a fixed-order gather) or 'rows' (one workgroup or thread per node row, no
         # workspace)
         self.jacobian_kernel = os.environ.get('GM_JACOBIAN_KERNEL', 'bins')
+        # 3D u-p P2G (flags and consistent-mass right-hand side): per particle bin, then a
+        # fixed-order gather per node (True), or one thread per node over its candidates
+        self.binned_p2g = True
         # multigrid: rebuild the coarse Galerkin operators at the first solve of a load step
         # only (later Newton corrections refresh level 0 and keep them), unless a solve then
         # needs three times the iterations of the step's first
@@ -227,11 +230,20 @@ class Engine:
                 self.mvals = torch.zeros(self.cap, dtype=torch.float64, device=self.dev)
             mrhs = torch.zeros(self.n_nodes, dtype=torch.float64, device=self.dev)
         binned_mass = mass and self.dim == 3
-        L.check(lib.gm_p2g(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ctypes.byref(self.pattern),
-                           float(threshold), L.ptr(pbc_flag), float(node_h_last),
-                           float(p_mass), L.ptr(p_part), L.ptr(self.act), L.ptr(self.bflag),
-                           L.ptr(self.mvals) if mass and not binned_mass else None, L.ptr(mrhs), _stream()),
-                'gm_p2g')
+        if self.dim == 3 and self.nb == 4 and self.binned_p2g:
+            # flags and right-hand side per particle bin, then a fixed-order gather per node
+            g = L.ctypes.byref(self.grid)
+            need = int(lib.gm_p2g_work_bytes(g))
+            w = self._bins_workspace(need)
+            L.check(lib.gm_p2g_binned(g, L.ctypes.byref(self.bins), float(threshold), L.ptr(pbc_flag),
+                                      float(node_h_last), float(p_mass), L.ptr(p_part), L.ptr(self.act),
+                                      L.ptr(self.bflag), L.ptr(mrhs), L.ptr(w), need, _stream()), 'gm_p2g_binned')
+        else:
+            L.check(lib.gm_p2g(L.ctypes.byref(self.grid), L.ctypes.byref(self.bins), L.ctypes.byref(self.pattern),
+                               float(threshold), L.ptr(pbc_flag), float(node_h_last),
+                               float(p_mass), L.ptr(p_part), L.ptr(self.act), L.ptr(self.bflag),
+                               L.ptr(self.mvals) if mass and not binned_mass else None, L.ptr(mrhs), _stream()),
+                    'gm_p2g')
         if binned_mass:
             g = L.ctypes.byref(self.grid)
             need = int(lib.gm_mass_work_bytes(g))

@@ -154,6 +154,18 @@ size_t gm_mass_work_bytes(const gm_grid* g);
 int gm_mass_binned(const gm_grid* g, const gm_bins* bins, const gm_pattern* pat, double p_mass, double* mvals,
                    void* work, size_t work_bytes, hipStream_t stream);
 
+/* 3D u-p P2G without the mass matrix (P2G_coupled_3d, mpm_utils.py:260-386: activation
+   where w_I > threshold, the pressure-boundary dof of every node at or above a flagged
+   particle among the 27 of its stencil, and mrhs_I = sum_p w_I m_p p_p sum_J w_J), the
+   same flags and right-hand side as gm_p2g with mvals null, summed per particle bin into
+   `work` (at least gm_p2g_work_bytes(g): n_nodes x 27 doubles and bytes) and gathered per
+   node in a fixed order.  act and bflag are only set, as by gm_p2g; mrhs may be null
+   (flags only), else p_part (particle pressures by particle id) is required. */
+size_t gm_p2g_work_bytes(const gm_grid* g);
+int gm_p2g_binned(const gm_grid* g, const gm_bins* bins, double threshold, const uint8_t* pbc_flag,
+                  double node_h_last, double p_mass, const double* p_part, uint8_t* act, uint8_t* bflag,
+                  double* mrhs, void* work, size_t work_bytes, hipStream_t stream);
+
 /* Preconditioner copy of the 3D u-p Newton matrix: adds coef * w_I * w_J to
    the zz entry of block (I, J) for every
    particle the indenter penalises (z >= height inside the footing box

@@ -137,3 +137,32 @@ def test_binned_jacobian_variants_write_the_same_bits(precision):
     assert 0 <= bb < 4 and 0 <= bg < 3
     got = eng.vals64 if (v32 and not t32) else eng.vals
     assert torch.equal(got[:n], ref.to(got.dtype))
+
+
+@pytest.mark.gpu
+def test_binned_p2g_matches_node_p2g():
+    """The binned 3D P2G (gm_p2g_binned: per bin, then a fixed-order gather per node) sets
+    the same activation and pressure-boundary flags as the node-centric gm_p2g, and its
+    consistent-mass right-hand side sums the same terms in another order (FP64 round-off),
+    on the deformed state with flagged surface particles."""
+    import torch
+    s = _deformed('fp64')
+    eng = s.engine
+    pbc = s.particle_pressure_boundary_flag_array.to(torch.uint8)
+    assert int(pbc.sum()) > 0
+    bflag0 = eng.bflag.clone()
+    bflag0.view(-1, 4)[:, 3] = 0  # the pressure dofs: set again by the P2G's boundary rule
+    out = {}
+    for binned in (False, True):
+        eng.binned_p2g = binned
+        eng.bflag.copy_(bflag0)
+        mrhs = eng.p2g(s.x_particles, s.GIMP_lp, 1e-7, pbc, s.dz, s.p_mass, s.particle_pressure_array, mass=True)
+        torch.cuda.synchronize()
+        out[binned] = (eng.act.clone(), eng.bflag.clone(), mrhs.clone(), eng.freem.clone())
+    eng.binned_p2g = True
+    (a0, f0, r0, m0), (a1, f1, r1, m1) = out[False], out[True]
+    rel = float((r1 - r0).norm() / r0.norm())
+    print('active dofs', int(a0.sum()), 'pressure-boundary dofs set', int((f0 != bflag0).sum()), 'rhs rel', rel)
+    assert int(a0.sum()) > 0 and int((f0 != bflag0).sum()) > 0
+    assert torch.equal(a0, a1) and torch.equal(f0, f1) and torch.equal(m0, m1)
+    assert float(r0.norm()) > 0 and rel < 1e-14
