@@ -112,6 +112,7 @@ _SIGS = {
     'gm_p2g': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), ctypes.POINTER(gm_pattern),
                        c_double, _P, c_double, c_double, _P, _P, _P, _P, _P, _P]),
     'gm_p2g_work_bytes': (ctypes.c_size_t, [ctypes.POINTER(gm_grid)]),
+    'gm_pattern_kernel': (c_int, [c_int]),
     'gm_p2g_binned': (c_int, [ctypes.POINTER(gm_grid), ctypes.POINTER(gm_bins), c_double, _P, c_double, c_double,
                               _P, _P, _P, _P, _P, ctypes.c_size_t, _P]),
     'gm_free_mask': (c_int, [c_int, _P, _P, _P, _P]),

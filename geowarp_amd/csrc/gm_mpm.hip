@@ -308,6 +308,66 @@ __global__ void k_pattern_count(gm_grid g, const int* cell_start, const uint8_t*
   row_nnzb[I] = __popcll(m0) + __popcll(m1);
 }
 
+// The same row masks from each bin's set of support shapes: bit e of extset[b] is set
+// when a particle of bin b has ext value e.  A particle's contribution to a row depends
+// only on its bin (base node) and its ext value, so the union over the bin's particles
+// is the union over the shapes present, at most 2^DIM per bin instead of one per particle.
+__global__ void k_bin_extset(int n_bins, const int* cell_start, const uint8_t* ext_s, int* extset) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_bins) return;
+  int s = 0;
+  for (int k = cell_start[b]; k < cell_start[b + 1]; ++k) s |= 1 << ext_s[k];
+  extset[b] = s;
+}
+
+template <int DIM>
+__global__ void k_pattern_count_sets(gm_grid g, const int* extset, uint64_t* mask, int* row_nnzb) {
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= g.n_nodes) return;
+  int ci[3] = {0, 0, 0};
+  node_coords<DIM>(g, I, ci);
+  const int ncell[3] = {g.nx, g.ny, g.nz};
+  uint64_t m0 = 0, m1 = 0;
+  int cb = (DIM == 2) ? (2 + 5 * 2) : (2 + 5 * 2 + 25 * 2);
+  if (cb < 64) m0 |= (1ull << cb); else m1 |= (1ull << (cb - 64));
+  const int oz_n = (DIM == 3) ? 3 : 1;
+  for (int oz = 0; oz < oz_n; ++oz)
+    for (int oy = 0; oy < 3; ++oy)
+      for (int ox = 0; ox < 3; ++ox) {
+        const int o[3] = {ox, oy, oz};
+        int bc[3] = {0, 0, 0};
+        bool ok = true;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+          bc[a] = ci[a] - o[a];
+          if (bc[a] < 0 || bc[a] > ncell[a]) ok = false;
+        }
+        if (!ok) continue;
+        const int es = extset[node_index<DIM>(g, bc)];
+        for (int ex = 0; ex < (1 << DIM); ++ex) {
+          if (!((es >> ex) & 1)) continue;
+          int e[3] = {1, 1, 1};
+          bool in = true;
+#pragma unroll
+          for (int a = 0; a < DIM; ++a) {
+            e[a] = ((ex >> a) & 1) ? 3 : 2;
+            in = in && o[a] < e[a];
+          }
+          if (!in) continue;
+          for (int kz = 0; kz < e[2]; ++kz)
+            for (int ky = 0; ky < e[1]; ++ky)
+              for (int kx = 0; kx < e[0]; ++kx) {
+                int rx = kx - o[0] + 2, ry = ky - o[1] + 2, rz = (DIM == 3) ? (kz - o[2] + 2) : 0;
+                int bit = rx + 5 * ry + 25 * rz;
+                if (bit < 64) m0 |= (1ull << bit); else m1 |= (1ull << (bit - 64));
+              }
+        }
+      }
+  mask[2 * I] = m0;
+  mask[2 * I + 1] = m1;
+  row_nnzb[I] = __popcll(m0) + __popcll(m1);
+}
+
 template <int DIM>
 __global__ void k_pattern_fill(gm_grid g, const uint64_t* mask, const int* row_ptr, int* col_idx) {
   int I = blockIdx.x * blockDim.x + threadIdx.x;
@@ -325,16 +385,32 @@ __global__ void k_pattern_fill(gm_grid g, const uint64_t* mask, const int* row_p
   }
 }
 
+// Row masks from the bins' support-shape sets (1, the default) or particle by particle (0).
+static int g_pattern_by_sets = 1;
+
+extern "C" int gm_pattern_kernel(int by_sets) {
+  g_pattern_by_sets = by_sets ? 1 : 0;
+  return 0;
+}
+
 // row_nnzb: n_nodes + 1 ints of scratch.  On return pat->nnzb holds the block count
 // (one host synchronisation, once per load step).
 extern "C" int gm_bsr_pattern(const gm_grid* g, const gm_bins* b, gm_pattern* pat, int* row_nnzb,
                               int max_nnzb, hipStream_t s) {
   GM_REQUIRE(g && b && pat, "gm_bsr_pattern: null argument");
   int n = g->n_nodes;
-  if (g->dim == 2)
+  if (g_pattern_by_sets) {
+    // the bins' support-shape sets in the binning's scratch counts (free once the particles are sorted)
+    k_bin_extset<<<gm_blocks(n, 256), 256, 0, s>>>(n, b->cell_start, b->ext_s, b->cell_count);
+    if (g->dim == 2)
+      k_pattern_count_sets<2><<<gm_blocks(n, 128), 128, 0, s>>>(*g, b->cell_count, pat->row_mask, row_nnzb);
+    else
+      k_pattern_count_sets<3><<<gm_blocks(n, 128), 128, 0, s>>>(*g, b->cell_count, pat->row_mask, row_nnzb);
+  } else if (g->dim == 2) {
     k_pattern_count<2><<<gm_blocks(n, 128), 128, 0, s>>>(*g, b->cell_start, b->ext_s, pat->row_mask, row_nnzb);
-  else
+  } else {
     k_pattern_count<3><<<gm_blocks(n, 128), 128, 0, s>>>(*g, b->cell_start, b->ext_s, pat->row_mask, row_nnzb);
+  }
   GM_LAUNCH_CHECK();
   int rc = exclusive_scan(row_nnzb, pat->row_ptr, n, b->scan_tmp, s);
   if (rc) return rc;

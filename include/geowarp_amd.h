@@ -61,6 +61,11 @@ int gm_bin_particles(const gm_grid* g, const double* x, const double* lp, gm_bin
 int gm_bsr_pattern(const gm_grid* g, const gm_bins* bins, gm_pattern* pat, int* row_nnzb, int max_nnzb,
                    hipStream_t stream);
 
+/* How gm_bsr_pattern forms its row masks (no counterpart in the reference): 1 (the
+   default) from each bin's set of particle support shapes, held in bins->cell_count
+   (the binning's scratch); 0 particle by particle.  Both give the same pattern. */
+int gm_pattern_kernel(int by_sets);
+
 /* Per-particle stage of one residual evaluation: gather the nodal increment
    (and pressures), update F, b_e, Cauchy stress, hardening variable; write the
    particle stress vector Y and (want_tangent) its analytic tangent D, both in

@@ -166,3 +166,38 @@ def test_binned_p2g_matches_node_p2g():
     assert int(a0.sum()) > 0 and int((f0 != bflag0).sum()) > 0
     assert torch.equal(a0, a1) and torch.equal(f0, f1) and torch.equal(m0, m1)
     assert float(r0.norm()) > 0 and rel < 1e-14
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dim', [2, 3])
+def test_pattern_from_support_sets_equals_particle_pattern(dim):
+    """gm_bsr_pattern's row masks from each bin's set of support shapes (the default) equal
+    those formed particle by particle, bit for bit, on a deformed state (stretched domains:
+    2- and 3-node supports)."""
+    import torch
+    import geowarp_amd.solvers as gs
+    from geowarp_amd import _lib as L
+    if dim == 3:
+        s = _deformed('fp64')
+    else:
+        cfg = H.MINI['bar_dp']()
+        s = H.make(gs, cfg)
+        s.advance_one_step(0, cfg['total_steps'])
+        s.prepare_step(1, cfg['total_steps'])
+    eng = s.engine
+    lib = L.lib()
+    out = {}
+    try:
+        for by_sets in (0, 1):
+            L.check(lib.gm_pattern_kernel(by_sets), 'gm_pattern_kernel')
+            eng.begin_step(s.x_particles, s.GIMP_lp)
+            torch.cuda.synchronize()
+            n = eng.n_nodes
+            out[by_sets] = (eng.row_mask.clone(), eng.row_ptr[:n + 1].clone(), eng.col_idx[:eng.nnzb].clone())
+    finally:
+        lib.gm_pattern_kernel(1)
+    ext = eng.ext[:s.x_particles.shape[0]]
+    print(dim, 'nnzb', eng.nnzb, 'particles with a 3-node axis', int((ext != 0).sum()), 'of', ext.numel())
+    assert int((ext != 0).sum()) > 0
+    for a, b in zip(out[0], out[1]):
+        assert torch.equal(a, b)
