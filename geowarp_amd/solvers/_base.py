@@ -40,6 +40,8 @@ Extensions over the reference (all optional, defaults reproduce it):
                                   (FP64 assembly, FP32 copy for the Krylov solve, refinement
                                   residual against the FP64 Jacobian)
   ['abs_floor']                   the Newton test also passes once the residual norm is below it
+  ['mass_tol'] (coupled)          relative tolerance of the consistent-mass projection (1e-3 x a
+                                  relative Newton tolerance within [1e-13, 1e-11]; else 1e-13)
   ['substeps'] (3D u-p)           levels of load-step halving after a failed Newton iteration
   iteration_dict['verbose']       print residual norms like the reference
   material_dict['H']              J2 linear isotropic hardening modulus
@@ -184,6 +186,11 @@ class _QuasiStaticBase:
         self.final_forcing = it.get('final_forcing', None if plastic else 1e-1)
         # absolute floor of the Newton test (None: the tolerance alone)
         self.abs_floor = it.get('abs_floor')
+        # coupled forms: relative tolerance of the consistent-mass projection, a thousand times
+        # tighter than a relative Newton tolerance, within [1e-13, 1e-11] (1e-13 under an
+        # absolute test); the reference solves it to 1e-8 (qc2d:293, qc3d:412)
+        self.mass_tol = float(it.get('mass_tol', max(1e-13, min(1e-11, 1e-3 * self.tol))
+                                     if self.tol_mode == 'relative' else 1e-13))
         # the last correction (its residual already passes the Newton test) solved on the
         # previous iteration's Jacobian, a chord step: the mixed form's default for elastic
         # materials, whose Jacobian barely moves over the last Newton correction

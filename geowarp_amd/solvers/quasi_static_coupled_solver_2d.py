@@ -94,7 +94,7 @@ class quasi_static_coupled_solver_2d(_QuasiStaticBase):
         mrhs = eng.p2g(self.x_particles, self.GIMP_lp, 1e-6, pbc, self.dy, self.p_mass, self.particle_pressure_array,
                        mass=True)
         self._sync_flags_out()
-        xp = eng.solve_mass(mrhs, x0=x0)
+        xp = eng.solve_mass(mrhs, tol=self.mass_tol, x0=x0)
         eng.set_nodal_pressure(xp, self._old, self._new, require_active=False)
 
     def advance_one_step(self, current_step, total_steps):

@@ -371,7 +371,7 @@ class quasi_static_coupled_solver_3d(_QuasiStaticBase):
         mrhs = eng.p2g(self.x_particles, self.GIMP_lp, 1e-7, pbc, self.dz, self.p_mass, self.particle_pressure_array,
                        mass=True)
         self._sync_flags_out()
-        xp = eng.solve_mass(mrhs, x0=x0)
+        xp = eng.solve_mass(mrhs, tol=self.mass_tol, x0=x0)
         eng.set_nodal_pressure(xp, self._old, self._new, require_active=True)
 
     def advance_one_step(self, current_step, total_steps):
