@@ -124,3 +124,35 @@ def test_chord_step_and_skipped_trial_tangent():
     assert na > 0 and nb == na
     assert cb['plain'] > 0 and cb['plain'] >= ca['plain'] and cb['tangent'] >= ca['tangent']
     assert torch.equal(xa, xb) and torch.equal(sa, sb) and torch.equal(pa, pb)
+
+
+@pytest.mark.gpu
+def test_mixed_form_fields_at_the_benchmark_tolerance():
+    """The benchmark's indentation (Neo-Hookean, indenter from the surface, physical penalty,
+    line search) on a 16 x 16 x 8-cell block, load steps 0-7 to 1e-8 relative residual with
+    the default options of each form (the mixed form's Eisenstat-Walker forcing, chord last
+    correction and kept coarse operators; the FP64 form's fixed 1e-10 inner tolerance): the
+    mixed form's displacement, stress and pore pressure stay within 1e-6 of the FP64 form's."""
+    import torch
+    import geowarp_amd.solvers as gs
+    from geowarp_amd import problems as P
+    out = {}
+    for form in ('fp64', 'mixed'):
+        cfg = P.indent3d(n_cells=(16, 16, 8), dx=1.0 / 64, total_steps=40, depth=0.05, footing=0.05,
+                         material='Neo-Hookean', precision=form, tol=1e-8, n_iter=12, dp_apex='return',
+                         shape_factor=0.1, footing_start='surface', penalty_form='physical')
+        s = P.build_solver(gs, cfg, device='cuda:0')
+        cfg['particle_flags'](s.x_particles, s.particle_traction_flag_array, s.particle_pressure_boundary_flag_array)
+        s.line_search = True
+        for k in range(8):
+            s.advance_one_step(k, cfg['total_steps'])
+            assert s.newton_converged, (form, k)
+        torch.cuda.synchronize()
+        out[form] = (s.x_particles - s.x0, s.particle_Cauchy_stress_array.clone(), s.particle_pressure_array.clone(),
+                     s.engine.stats.get('chord_steps', 0))
+    (u0, s0, p0, _), (u1, s1, p1, chords) = out['fp64'], out['mixed']
+    rel = [float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) for a, b in ((u1, u0), (s1, s0), (p1, p0))]
+    print('mixed vs FP64 after 8 load steps: displacement %.2e stress %.2e pressure %.2e, chord steps %d'
+          % (rel[0], rel[1], rel[2], chords))
+    assert chords > 0
+    assert max(rel) < 1e-6
