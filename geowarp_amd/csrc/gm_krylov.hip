@@ -193,26 +193,7 @@ __device__ __forceinline__ void row_acc(int row, int lane, int n_rows, const int
                                         const TM* vals, const TX* x, double acc[NB]) {
 #pragma unroll
   for (int r = 0; r < NB; ++r) acc[r] = 0.0;
-  if (NB == 1 && row < n_rows) {
-    // scalar rows: four consecutive entries per lane and step (the row's lanes read one
-    // contiguous stretch of 4 G entries), their loads issued before they are accumulated
-    const int s = row_ptr[row], e = row_ptr[row + 1];
-    for (int k = s + 4 * lane; k < e; k += 4 * G) {
-      int c[4];
-      TM v[4];
-      double xv[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const bool in = k + u < e;
-        c[u] = in ? col_idx[k + u] : -1;
-        v[u] = in ? vals[k + u] : (TM)0;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) xv[u] = c[u] >= 0 ? (double)x[c[u]] : 0.0;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) acc[0] += (double)v[u] * xv[u];
-    }
-  } else if (row < n_rows) {
+  if (row < n_rows) {
     int s = row_ptr[row], e = row_ptr[row + 1];
     for (int k = s + lane; k < e; k += G) {
       int c = col_idx[k];
