@@ -186,6 +186,60 @@ __global__ void __launch_bounds__(T) k_lo_convert(int n_rows, const int* row_ptr
     for (int q = 0; q < NB * NB; ++q) lo[(long)k * NB * NB + q] = (gm_half)((float)vals[(long)k * NB * NB + q] * inv[q]);
 }
 
+// The same conversion with G lanes per row and up to C blocks per lane kept in registers
+// between the row maximum and the conversion, so that a row of at most G C blocks is read
+// from memory once (the FP16 values and scales are those of k_lo_convert).
+template <int NB, int G, int T, typename TM, int C>
+__global__ void __launch_bounds__(T) k_lo_convert_cached(int n_rows, const int* row_ptr, const TM* vals, gm_half* lo,
+                                                         float* scale) {
+  const int row = blockIdx.x * (T / G) + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  float mx[NB * NB], cv[C][NB * NB];
+#pragma unroll
+  for (int q = 0; q < NB * NB; ++q) mx[q] = 0.0f;
+  int s = 0, e = 0;
+  if (row < n_rows) {
+    s = row_ptr[row];
+    e = row_ptr[row + 1];
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+      const int k = s + lane + u * G;
+      if (k < e) {
+#pragma unroll
+        for (int q = 0; q < NB * NB; ++q) {
+          cv[u][q] = (float)vals[(long)k * NB * NB + q];
+          mx[q] = fmaxf(mx[q], fabsf(cv[u][q]));
+        }
+      }
+    }
+    for (int k = s + lane + C * G; k < e; k += G)
+#pragma unroll
+      for (int q = 0; q < NB * NB; ++q) mx[q] = fmaxf(mx[q], fabsf((float)vals[(long)k * NB * NB + q]));
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int q = 0; q < NB * NB; ++q) mx[q] = fmaxf(mx[q], __shfl_xor(mx[q], off, G));
+  if (row >= n_rows) return;
+  float inv[NB * NB];
+#pragma unroll
+  for (int q = 0; q < NB * NB; ++q) {
+    float sc = (mx[q] > 0.0f && mx[q] < 3.0e38f) ? mx[q] : 1.0f;
+    inv[q] = 1.0f / sc;
+    if (lane == 0) scale[(long)row * NB * NB + q] = sc;
+  }
+#pragma unroll
+  for (int u = 0; u < C; ++u) {
+    const int k = s + lane + u * G;
+    if (k < e)
+#pragma unroll
+      for (int q = 0; q < NB * NB; ++q) lo[(long)k * NB * NB + q] = (gm_half)(cv[u][q] * inv[q]);
+  }
+  for (int k = s + lane + C * G; k < e; k += G)
+#pragma unroll
+    for (int q = 0; q < NB * NB; ++q) lo[(long)k * NB * NB + q] = (gm_half)((float)vals[(long)k * NB * NB + q] * inv[q]);
+}
+
 // Row product of a level matrix, 16 (8, 4) lanes per block row; every lane of the row
 // ends with the row's full product (butterfly reduction in a fixed order).
 template <int NB, int G, typename TM, typename TX>
@@ -312,8 +366,14 @@ __global__ void __launch_bounds__(256) k_level(int n_rows, const int* row_ptr, c
 template <int NB, typename TM>
 static void launch_lo_convert(int n_rows, const int* rp, const TM* vals, gm_half* lo, float* scale,
                               hipStream_t s) {
-  constexpr int G = spmv_lanes(NB);
-  k_lo_convert<NB, G, 256, TM><<<gm_blocks(n_rows, 256 / G), 256, 0, s>>>(n_rows, rp, vals, lo, scale);
+  if constexpr (NB == 4 && sizeof(TM) == 4) {
+    // 3D u-p rows hold at most 125 blocks: 32 lanes with 4 blocks each read every row once
+    k_lo_convert_cached<NB, 32, 256, TM, 4><<<gm_blocks(n_rows, 256 / 32), 256, 0, s>>>(n_rows, rp, vals, lo,
+                                                                                        scale);
+  } else {
+    constexpr int G = spmv_lanes(NB);
+    k_lo_convert<NB, G, 256, TM><<<gm_blocks(n_rows, 256 / G), 256, 0, s>>>(n_rows, rp, vals, lo, scale);
+  }
 }
 
 // workgroups of a launch with fused dot products (SPMV_T threads): the partial-sum count
