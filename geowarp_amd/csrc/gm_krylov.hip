@@ -443,11 +443,25 @@ __global__ void k_diag_inv(int n_rows, const int* row_ptr, const int* col_idx, c
   if (row >= n_rows) return;
   int s = row_ptr[row], e = row_ptr[row + 1];
   int k = -1;
-  for (int q = s; q < e; ++q)
-    if (col_idx[q] == row) {
-      k = q;
+  // binary search (the patterns list a row's columns in ascending order), then a linear
+  // scan should a row not be sorted
+  for (int lo = s, hi = e - 1; lo <= hi;) {
+    const int mid = (lo + hi) >> 1, c = col_idx[mid];
+    if (c == row) {
+      k = mid;
       break;
     }
+    if (c < row)
+      lo = mid + 1;
+    else
+      hi = mid - 1;
+  }
+  if (k < 0)
+    for (int q = s; q < e; ++q)
+      if (col_idx[q] == row) {
+        k = q;
+        break;
+      }
   double M[NB][2 * NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i)
