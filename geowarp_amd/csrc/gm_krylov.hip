@@ -1148,14 +1148,30 @@ extern "C" int gm_d2f(long n, const double* a, float* b, hipStream_t s) {
 __global__ void k_fix_empty_diag(int n_rows, const int* row_ptr, const int* col_idx, double* vals, double* rhs) {
   int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n_rows) return;
-  for (int q = row_ptr[row]; q < row_ptr[row + 1]; ++q)
-    if (col_idx[q] == row) {
-      if (vals[q] == 0.0) {
-        vals[q] = 1.0;
-        rhs[row] = 0.0;
-      }
-      return;
+  const int s = row_ptr[row], e = row_ptr[row + 1];
+  int k = -1;
+  // binary search (ascending columns), then a linear scan should a row not be sorted
+  for (int lo = s, hi = e - 1; lo <= hi;) {
+    const int mid = (lo + hi) >> 1, c = col_idx[mid];
+    if (c == row) {
+      k = mid;
+      break;
     }
+    if (c < row)
+      lo = mid + 1;
+    else
+      hi = mid - 1;
+  }
+  if (k < 0)
+    for (int q = s; q < e; ++q)
+      if (col_idx[q] == row) {
+        k = q;
+        break;
+      }
+  if (k >= 0 && vals[k] == 0.0) {
+    vals[k] = 1.0;
+    rhs[row] = 0.0;
+  }
 }
 extern "C" int gm_fix_empty_diag(int n_rows, const int* row_ptr, const int* col_idx, double* vals, double* rhs,
                                  hipStream_t s) {
