@@ -1527,18 +1527,19 @@ static int gmres_impl(int n_rows, const int* rp, const int* ci, const TM* vals, 
   double rel = 1.0;
   bool first = true;
   while (total < maxiter) {
-    // r = b - A x  -> V[0]
+    // r = b - A x  -> V[0]; on the first cycle x = 0, so r = b and ||r||^2 is the ||b||^2
+    // already in hb (the same reduction of the same values)
     if (first) {
       GM_HIP(hipMemcpyAsync(V, b, sizeof(TV) * n, hipMemcpyDeviceToDevice, st));
     } else {
       launch_spmv<NB, TM, TV, TV>(n_rows, rp, ci, vals, x, w, nullptr, nullptr, nullptr, st);
       k_sub_into<TV><<<vb, RED_T, 0, st>>>(n, b, w, V);
+      k_dot2<TV><<<vb, RED_T, 0, st>>>(n, V, V, nullptr, nullptr, part);
+      k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh, nullptr);
+      GM_LAUNCH_CHECK();
+      GM_HIP(hipMemcpyAsync(&hb, dh, sizeof(double), hipMemcpyDeviceToHost, st));
+      GM_HIP(hipStreamSynchronize(st));
     }
-    k_dot2<TV><<<vb, RED_T, 0, st>>>(n, V, V, nullptr, nullptr, part);
-    k_sum_parts<<<1, RED_T, 0, st>>>(part, vb, dh, nullptr);
-    GM_LAUNCH_CHECK();
-    GM_HIP(hipMemcpyAsync(&hb, dh, sizeof(double), hipMemcpyDeviceToHost, st));
-    GM_HIP(hipStreamSynchronize(st));
     double beta = sqrt(hb);
     rel = beta / bnorm;
     if (!(rel == rel)) {
