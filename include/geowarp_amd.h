@@ -15,8 +15,11 @@
  *     success or a negative code; gm_last_error() then describes the failure;
  *   - the library allocates no device memory: buffers and workspaces are
  *     passed in (the Python layer allocates them with torch);
- *   - calls that return a host value (gm_bsr_pattern's block count,
- *     gm_bicgstab's iteration count) synchronise `stream`.
+ *   - calls that return a host value (gm_bsr_pattern's block count, the
+ *     Krylov solvers' iteration counts and residuals) synchronise `stream`;
+ *     the GMRES solvers also read each Arnoldi step's Hessenberg column on
+ *     the host (Givens rotations and the stopping test run there), and CG
+ *     reads its state every `check_every` iterations.
  *   The argument structs gm_grid, gm_bins, gm_pattern, gm_stage_args and
  *   gm_krylov_ws are defined in geowarp_amd/csrc/gm_types.h.
  */
